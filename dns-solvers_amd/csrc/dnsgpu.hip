@@ -1,0 +1,686 @@
+// dnsgpu.hip -- host side of the C ABI declared in include/dnsgpu.h.
+//
+// One handle = one channel case resident on one GPU: two population
+// buffers (A-B pull streaming), the two wall masks, the partial-sum
+// buffers of the fixed-order reductions, a ring of per-step log rows and
+// one HIP stream.  The per-step sequence is
+//     step_kernel -> reduce_rows_kernel -> finalize_kernel
+// all on the handle's stream; the host only reads back log rows.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "../../include/dnsgpu.h"
+#include "lbm_io_kernels.h"
+#include "lbm_kernels.h"
+
+using namespace lbm;
+
+static thread_local char g_err[512] = "";
+
+static int set_err(int code, const char* fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return set_err(DNSGPU_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                           __FILE__, __LINE__);                                               \
+    } while (0)
+
+struct dnsgpu {
+    dns_case c;
+    Geom g;
+    int esize;  // bytes per stored population
+    void* buf[2];
+    int cur;         // buffer holding the current state
+    int poststream;  // 1: buf[cur] holds post-stream populations (after load/init)
+    int8_t* mbot;
+    int8_t* mtop;
+    int16_t* hbot;
+    int16_t* htop;
+    double* part;     // [kNQ][Z][nblk]
+    double* gpart;    // [kNG][nblk]
+    double* planes;   // [kNQ*Z + kNG] reduced sums (planes then globals)
+    double* shift;    // [3][Z]
+    double* means;    // [4][Z] statistics pass-1 means
+    DevState* st;
+    LogRow* ring;
+    int ring_cap;
+    int nblk;
+    int nblk_stats;
+    int block;
+    hipStream_t stream;
+    double omega, mu, dt;
+    double* stage;
+    size_t stage_bytes;
+    size_t dev_bytes;
+    hipEvent_t ev0, ev1;
+};
+
+extern "C" const char* dnsgpu_last_error(void) { return g_err; }
+
+static long long plane_stride(long long n)
+{
+    // Round each direction plane up to 256 elements and stagger
+    // consecutive planes by 4 KiB of extra offset so the 19 load streams
+    // and 19 store streams of a wavefront do not start on the same HBM
+    // channel.  DNSGPU_PLANE_PAD (elements) overrides the stagger.
+    long long pad = 512;
+    const char* e = getenv("DNSGPU_PLANE_PAD");
+    if (e) pad = atoll(e);
+    return ((n + 255) / 256) * 256 + pad;
+}
+
+template <typename K>
+static int occupancy_blocks(K kernel, int block, int* out)
+{
+    int dev = 0, ncu = 0, per = 0;
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, block, 0));
+    if (per < 1) per = 1;
+    *out = per * ncu;
+    return 0;
+}
+
+static int stage_reserve(dnsgpu* h, size_t bytes)
+{
+    if (h->stage_bytes >= bytes) return 0;
+    if (h->stage) (void)hipFree(h->stage);
+    h->stage = nullptr;
+    h->stage_bytes = 0;
+    if (hipMalloc(&h->stage, bytes) != hipSuccess)
+        return set_err(DNSGPU_ENOMEM, "staging allocation of %zu bytes failed", bytes);
+    h->stage_bytes = bytes;
+    return 0;
+}
+
+static StepArgs step_args(dnsgpu* h, const void* src, void* dst, int nblk)
+{
+    StepArgs A;
+    A.g = h->g;
+    A.src = src;
+    A.dst = dst;
+    A.mbot = h->mbot;
+    A.mtop = h->mtop;
+    A.omega = h->omega;
+    A.G = &h->st->G;
+    A.shift = h->shift;
+    A.part = h->part;
+    A.gpart = h->gpart;
+    A.nblk = nblk;
+    return A;
+}
+
+static FinalizeArgs fin_args(dnsgpu* h, bool stats, LogRow* row, long long cntr, int from_moments)
+{
+    FinalizeArgs F;
+    F.planes = stats ? h->planes : nullptr;
+    F.globals = h->planes + (size_t)kNQ * h->g.Z;
+    F.shift = h->shift;
+    F.st = h->st;
+    F.row = row;
+    F.X = h->g.X;
+    F.Y = h->g.Y;
+    F.Z = h->g.Z;
+    F.forcing = h->c.forcing;
+    F.ub = h->c.ub;
+    F.F = h->c.force;
+    F.mu = h->mu;
+    F.TM = (double)h->c.TSTR;
+    F.dt = h->dt;
+    F.cntr = cntr;
+    F.from_moments = from_moments;
+    return F;
+}
+
+// Partials -> plane sums and globals in h->planes (fixed order).
+static int reduce_partials(dnsgpu* h, bool stats, int nblk)
+{
+    const int Z = h->g.Z;
+    if (stats)
+        hipLaunchKernelGGL(reduce_rows_kernel, dim3(kNQ * Z), dim3(256), 0, h->stream, h->part, nblk, h->planes);
+    hipLaunchKernelGGL(reduce_rows_kernel, dim3(kNG), dim3(256), 0, h->stream, h->gpart, nblk,
+                       h->planes + (size_t)kNQ * Z);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+template <typename T>
+static int launch_step(dnsgpu* h, bool stats, LogRow* row, long long cntr)
+{
+    const void* src = h->buf[h->cur];
+    void* dst = h->buf[1 - h->cur];
+    StepArgs A = step_args(h, src, dst, h->nblk);
+    dim3 grid(h->nblk), block(h->block);
+    if (h->poststream) {
+        if (stats)
+            hipLaunchKernelGGL((step_kernel<T, true, true>), grid, block, 0, h->stream, A);
+        else
+            hipLaunchKernelGGL((step_kernel<T, true, false>), grid, block, 0, h->stream, A);
+    } else {
+        if (stats)
+            hipLaunchKernelGGL((step_kernel<T, false, true>), grid, block, 0, h->stream, A);
+        else
+            hipLaunchKernelGGL((step_kernel<T, false, false>), grid, block, 0, h->stream, A);
+    }
+    HIPCHK(hipGetLastError());
+    int rc = reduce_partials(h, stats, h->nblk);
+    if (rc) return rc;
+    FinalizeArgs F = fin_args(h, stats, row, cntr, 0);
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1024), 0, h->stream, F);
+    HIPCHK(hipGetLastError());
+    h->cur = 1 - h->cur;
+    h->poststream = 0;
+    return 0;
+}
+
+static int do_step(dnsgpu* h, bool stats, LogRow* row, long long cntr)
+{
+    if (h->c.precision == DNSGPU_FP64) return launch_step<double>(h, stats, row, cntr);
+    return launch_step<float>(h, stats, row, cntr);
+}
+
+// Derive the force of the next step and the plane-mean shifts from the
+// current state (encalc.c:29-121 on the loaded/initial state).
+template <typename T>
+static int launch_moments(dnsgpu* h)
+{
+    StepArgs A = step_args(h, h->buf[h->cur], nullptr, h->nblk);
+    if (h->poststream)
+        hipLaunchKernelGGL((moments_kernel<T, true>), dim3(h->nblk), dim3(h->block), 0, h->stream, A);
+    else
+        hipLaunchKernelGGL((moments_kernel<T, false>), dim3(h->nblk), dim3(h->block), 0, h->stream, A);
+    HIPCHK(hipGetLastError());
+    int rc = reduce_partials(h, true, h->nblk);
+    if (rc) return rc;
+    FinalizeArgs F = fin_args(h, true, nullptr, 0, 1);
+    hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1024), 0, h->stream, F);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static int refresh_force(dnsgpu* h)
+{
+    HIPCHK(hipMemsetAsync(h->shift, 0, sizeof(double) * 3 * h->g.Z, h->stream));
+    if (h->c.precision == DNSGPU_FP64) return launch_moments<double>(h);
+    return launch_moments<float>(h);
+}
+
+extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
+{
+    if (!c || !out) return set_err(DNSGPU_EARG, "null argument");
+    *out = nullptr;
+    if (c->xdv < 2 || c->ydv < 2 || c->zdv < 4 || c->zdv > kWave * kMaxWaves)
+        return set_err(DNSGPU_EARG, "grid %dx%dx%d unsupported (xdv,ydv>=2, 4<=zdv<=%d)", c->xdv, c->ydv, c->zdv,
+                       kWave * kMaxWaves);
+    if ((long long)c->xdv * c->ydv * c->zdv >= (1ll << 31))
+        return set_err(DNSGPU_EARG, "grid has more than 2^31 nodes");
+    if (c->precision != DNSGPU_FP64 && c->precision != DNSGPU_FP32)
+        return set_err(DNSGPU_EARG, "precision %d not built", c->precision);
+    dnsgpu* h = (dnsgpu*)calloc(1, sizeof(dnsgpu));
+    if (!h) return set_err(DNSGPU_ENOMEM, "host allocation failed");
+    h->c = *c;
+    Geom& g = h->g;
+    g.X = c->xdv;
+    g.Y = c->ydv;
+    g.Z = c->zdv;
+    g.nzc = (g.Z + kWave - 1) / kWave;
+    g.cpb = std::max(1, 4 / g.nzc);
+    g.ncols = g.X * g.Y;
+    g.ngroups = (g.ncols + g.cpb - 1) / g.cpb;
+    g.P = plane_stride((long long)g.X * g.Y * g.Z);
+    h->block = kWave * g.nzc * g.cpb;
+    h->esize = c->precision == DNSGPU_FP64 ? 8 : 4;
+    // omega and the constants exactly as dns-const-q-xycut.c:211-217,
+    // wshearstr.c:24, encalc.c:20 evaluate them.
+    const int zd = g.Z + 2;
+    if (c->omega > 0)
+        h->omega = c->omega;
+    else {
+        double tau = (3. / c->Rebs) * c->ub * (zd - 2.) + 0.5;
+        h->omega = 1. / tau;
+    }
+    h->mu = (1. / 6.) * (2. / h->omega - 1);
+    h->dt = 1. / ((double)(0.5 * (zd - 2.) / c->CFL));
+
+    int rc = 0;
+    auto fail = [&](int code) {
+        dnsgpu_destroy(h);
+        return code;
+    };
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_err(DNSGPU_EHIP, "stream creation failed"));
+    if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+        return fail(set_err(DNSGPU_EHIP, "event creation failed"));
+    // grid: fill the device once (persistent-style), at most one block per group
+    int occ = 0;
+    if (c->precision == DNSGPU_FP64)
+        rc = occupancy_blocks(step_kernel<double, false, true>, h->block, &occ);
+    else
+        rc = occupancy_blocks(step_kernel<float, false, true>, h->block, &occ);
+    if (rc) return fail(rc);
+    const char* ob = getenv("DNSGPU_BLOCKS");
+    if (ob) occ = atoi(ob);
+    h->nblk = std::max(1, std::min(occ, g.ngroups));
+
+    const size_t popbytes = (size_t)Q * g.P * h->esize;
+    const size_t xy = (size_t)g.X * g.Y;
+    struct {
+        void** p;
+        size_t n;
+    } allocs[] = {
+        {&h->buf[0], popbytes},
+        {&h->buf[1], popbytes},
+        {(void**)&h->mbot, xy},
+        {(void**)&h->mtop, xy},
+        {(void**)&h->hbot, xy * 2},
+        {(void**)&h->htop, xy * 2},
+        {(void**)&h->part, sizeof(double) * kNQ * g.Z * (size_t)h->nblk},
+        {(void**)&h->gpart, sizeof(double) * kNG * (size_t)h->nblk},
+        {(void**)&h->planes, sizeof(double) * (kNQ * g.Z + kNG)},
+        {(void**)&h->shift, sizeof(double) * 3 * g.Z},
+        {(void**)&h->means, sizeof(double) * 4 * g.Z},
+        {(void**)&h->st, sizeof(DevState)},
+        {(void**)&h->ring, sizeof(LogRow) * 1024},
+    };
+    for (auto& a : allocs) {
+        if (hipMalloc(a.p, a.n) != hipSuccess)
+            return fail(set_err(DNSGPU_ENOMEM, "device allocation of %zu bytes failed", a.n));
+        h->dev_bytes += a.n;
+    }
+    h->ring_cap = 1024;
+    if (hipMemsetAsync(h->mbot, -1, xy, h->stream) != hipSuccess ||
+        hipMemsetAsync(h->mtop, -1, xy, h->stream) != hipSuccess ||
+        hipMemsetAsync(h->hbot, 0, xy * 2, h->stream) != hipSuccess ||
+        hipMemsetAsync(h->htop, 0, xy * 2, h->stream) != hipSuccess ||
+        hipMemsetAsync(h->st, 0, sizeof(DevState), h->stream) != hipSuccess ||
+        hipMemsetAsync(h->shift, 0, sizeof(double) * 3 * g.Z, h->stream) != hipSuccess)
+        return fail(set_err(DNSGPU_EHIP, "initial memset failed"));
+    // the pop buffers start at rest
+    h->cur = 0;
+    h->poststream = 1;
+    if (hipStreamSynchronize(h->stream) != hipSuccess) return fail(set_err(DNSGPU_EHIP, "sync failed"));
+    rc = dnsgpu_init_rest(h);
+    if (rc) return fail(rc);
+    *out = h;
+    return 0;
+}
+
+extern "C" void dnsgpu_destroy(dnsgpu* h)
+{
+    if (!h) return;
+    if (h->stream) (void)hipStreamSynchronize(h->stream);
+    void* ptrs[] = {h->buf[0], h->buf[1], h->mbot, h->mtop, h->hbot,   h->htop, h->part,
+                    h->gpart,  h->planes, h->shift, h->means, h->st, h->ring, h->stage};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    free(h);
+}
+
+extern "C" size_t dnsgpu_device_bytes(const dnsgpu* h) { return h ? h->dev_bytes : 0; }
+
+extern "C" int dnsgpu_set_walls(dnsgpu* h, const int8_t* bottom, const int8_t* top)
+{
+    if (!h) return set_err(DNSGPU_EARG, "null handle");
+    const size_t xy = (size_t)h->g.X * h->g.Y;
+    const int8_t* src[2] = {bottom, top};
+    int8_t* dst[2] = {h->mbot, h->mtop};
+    for (int w = 0; w < 2; w++) {
+        if (src[w]) {
+            for (size_t i = 0; i < xy; i++)
+                if (src[w][i] != -1 && src[w][i] != -2)
+                    return set_err(DNSGPU_EARG, "wall code %d at %zu (must be -1 or -2)", src[w][i], i);
+            HIPCHK(hipMemcpyAsync(dst[w], src[w], xy, hipMemcpyHostToDevice, h->stream));
+        } else {
+            HIPCHK(hipMemsetAsync(dst[w], -1, xy, h->stream));
+        }
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    // the first step's force depends on the state only; the budget of later
+    // steps reads the masks directly.
+    return 0;
+}
+
+extern "C" int dnsgpu_set_solid_bands(dnsgpu* h, const int16_t* bottom, const int16_t* top)
+{
+    if (!h) return set_err(DNSGPU_EARG, "null handle");
+    const size_t xy = (size_t)h->g.X * h->g.Y;
+    bool any = false;
+    for (int w = 0; w < 2; w++) {
+        const int16_t* s = w ? top : bottom;
+        if (s)
+            for (size_t i = 0; i < xy; i++) any |= s[i] != 0;
+    }
+    if (any) return set_err(DNSGPU_EARG, "solid bands inside the fluid rows are not supported by this build");
+    return 0;
+}
+
+static int launch_init(dnsgpu* h, int kind, const dns_init* spec)
+{
+    InitArgs A;
+    memset(&A, 0, sizeof A);
+    A.g = h->g;
+    A.cur = h->buf[0];
+    A.other = h->buf[1];
+    A.kind = kind;
+    if (spec) {
+        A.uc = spec->uc;
+        A.amp = spec->amp;
+        A.seed = spec->seed;
+        A.nmodes = spec->nmodes;
+    }
+    A.nu = (1. / 3.) * (1. / h->omega - 0.5);
+    A.kappa_inv = 1.0 / 0.41;
+    A.Bplus = 5.2;
+    const long long n = (long long)h->g.X * h->g.Y * h->g.Z;
+    int grid = (int)std::min<long long>((n + 255) / 256, 65536);
+    if (h->c.precision == DNSGPU_FP64)
+        hipLaunchKernelGGL(init_kernel<double>, dim3(grid), dim3(256), 0, h->stream, A);
+    else
+        hipLaunchKernelGGL(init_kernel<float>, dim3(grid), dim3(256), 0, h->stream, A);
+    HIPCHK(hipGetLastError());
+    h->cur = 0;
+    h->poststream = 1;
+    int rc = refresh_force(h);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int dnsgpu_init_rest(dnsgpu* h)
+{
+    if (!h) return set_err(DNSGPU_EARG, "null handle");
+    return launch_init(h, 0, nullptr);
+}
+
+extern "C" int dnsgpu_init_field(dnsgpu* h, const dns_init* spec)
+{
+    if (!h || !spec) return set_err(DNSGPU_EARG, "null argument");
+    if (spec->kind < 0 || spec->kind > 2) return set_err(DNSGPU_EARG, "unknown init kind %d", spec->kind);
+    if (spec->kind == 2 && (spec->nmodes < 1 || spec->nmodes > 31))
+        return set_err(DNSGPU_EARG, "nmodes must be in [1, 31]");
+    return launch_init(h, spec->kind, spec);
+}
+
+static int check_slab(dnsgpu* h, int x0, int nx)
+{
+    if (x0 < 0 || nx < 1 || x0 + nx > h->g.X)
+        return set_err(DNSGPU_EARG, "slab [%d, %d) outside [0, %d)", x0, x0 + nx, h->g.X);
+    return 0;
+}
+
+extern "C" int dnsgpu_load_aos(dnsgpu* h, int x0, int nx, const double* ftemp, const double* f)
+{
+    if (!h || !ftemp || !f) return set_err(DNSGPU_EARG, "null argument");
+    int rc = check_slab(h, x0, nx);
+    if (rc) return rc;
+    const size_t half = (size_t)nx * h->g.Y * (h->g.Z + 2) * Q * sizeof(double);
+    if ((rc = stage_reserve(h, 2 * half))) return rc;
+    HIPCHK(hipMemcpyAsync(h->stage, ftemp, half, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync((char*)h->stage + half, f, half, hipMemcpyHostToDevice, h->stream));
+    AosArgs A;
+    memset(&A, 0, sizeof A);
+    A.g = h->g;
+    A.cur_w = h->buf[0];
+    A.other_w = h->buf[1];
+    A.x0 = x0;
+    A.nx = nx;
+    A.ftemp = h->stage;
+    A.f = (double*)((char*)h->stage + half);
+    const long long n = (long long)nx * h->g.Y * h->g.Z;
+    int grid = (int)std::min<long long>((n + 255) / 256, 65536);
+    if (h->c.precision == DNSGPU_FP64)
+        hipLaunchKernelGGL(load_aos_kernel<double>, dim3(grid), dim3(256), 0, h->stream, A);
+    else
+        hipLaunchKernelGGL(load_aos_kernel<float>, dim3(grid), dim3(256), 0, h->stream, A);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->cur = 0;
+    h->poststream = 1;
+    return 0;
+}
+
+extern "C" int dnsgpu_commit_load(dnsgpu* h)
+{
+    if (!h) return set_err(DNSGPU_EARG, "null handle");
+    h->cur = 0;
+    h->poststream = 1;
+    int rc = refresh_force(h);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int dnsgpu_save_aos(dnsgpu* h, int x0, int nx, double* ftemp, double* f)
+{
+    if (!h || !ftemp || !f) return set_err(DNSGPU_EARG, "null argument");
+    int rc = check_slab(h, x0, nx);
+    if (rc) return rc;
+    const size_t half = (size_t)nx * h->g.Y * (h->g.Z + 2) * Q * sizeof(double);
+    if ((rc = stage_reserve(h, 2 * half))) return rc;
+    AosArgs A;
+    memset(&A, 0, sizeof A);
+    A.g = h->g;
+    A.cur = h->buf[h->cur];
+    A.other = h->buf[1 - h->cur];
+    A.mbot = h->mbot;
+    A.mtop = h->mtop;
+    A.local = h->poststream;
+    A.x0 = x0;
+    A.nx = nx;
+    A.ftemp = h->stage;
+    A.f = (double*)((char*)h->stage + half);
+    const long long n = (long long)nx * h->g.Y * (h->g.Z + 2);
+    int grid = (int)std::min<long long>((n + 255) / 256, 65536);
+    if (h->c.precision == DNSGPU_FP64)
+        hipLaunchKernelGGL(save_aos_kernel<double>, dim3(grid), dim3(256), 0, h->stream, A);
+    else
+        hipLaunchKernelGGL(save_aos_kernel<float>, dim3(grid), dim3(256), 0, h->stream, A);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(ftemp, h->stage, half, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(f, (char*)h->stage + half, half, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int dnsgpu_get_force(dnsgpu* h, double* force)
+{
+    if (!h || !force) return set_err(DNSGPU_EARG, "null argument");
+    HIPCHK(hipMemcpyAsync(force, &h->st->G, sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+static int read_rows(dnsgpu* h, dns_steplog* rows, int n)
+{
+    static_assert(sizeof(LogRow) == sizeof(dns_steplog), "log row layout");
+    HIPCHK(hipMemcpyAsync(rows, h->ring, sizeof(LogRow) * n, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int i = 0; i < n; i++)
+        if (isnan(rows[i].stress))
+            return set_err(DNSGPU_ENAN, "Eroor: NAN, found in wall shear stress (step row %d)", i);
+    return 0;
+}
+
+extern "C" int dnsgpu_step(dnsgpu* h, long cntr0, int nsteps, dns_steplog* rows)
+{
+    if (!h) return set_err(DNSGPU_EARG, "null handle");
+    if (nsteps < 0) return set_err(DNSGPU_EARG, "nsteps < 0");
+    int done = 0;
+    while (done < nsteps) {
+        const int chunk = rows ? std::min(nsteps - done, h->ring_cap) : nsteps - done;
+        for (int s = 0; s < chunk; s++) {
+            int rc = do_step(h, rows != nullptr, rows ? h->ring + s : nullptr, (long long)cntr0 + done + s);
+            if (rc) return rc;
+        }
+        if (rows) {
+            int rc = read_rows(h, rows + done, chunk);
+            if (rc) return rc;
+        }
+        done += chunk;
+    }
+    if (!rows) {
+        double G;
+        int rc = dnsgpu_get_force(h, &G);
+        if (rc) return rc;
+        if (isnan(G)) return set_err(DNSGPU_ENAN, "Eroor: NAN, found in the force budget");
+    }
+    return 0;
+}
+
+template <typename T>
+static int launch_stats_pass(dnsgpu* h, int pass)
+{
+    StatArgs A;
+    A.g = h->g;
+    A.cur = h->buf[h->cur];
+    A.mbot = h->mbot;
+    A.mtop = h->mtop;
+    A.local = h->poststream;
+    A.pass = pass;
+    A.means = h->means;
+    A.part = h->part;
+    A.nblk = h->nblk;
+    hipLaunchKernelGGL(stats_kernel<T>, dim3(h->nblk), dim3(h->block), 0, h->stream, A);
+    HIPCHK(hipGetLastError());
+    hipLaunchKernelGGL(reduce_rows_kernel, dim3(5 * h->g.Z), dim3(256), 0, h->stream, h->part, h->nblk, h->planes);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int dnsgpu_statistics(dnsgpu* h, double* out)
+{
+    if (!h || !out) return set_err(DNSGPU_EARG, "null argument");
+    const int Z = h->g.Z, zd = Z + 2;
+    const double nxy = (double)(h->g.X * h->g.Y);
+    std::vector<double> s(5 * (size_t)Z), m(4 * (size_t)Z);
+    int rc = h->c.precision == DNSGPU_FP64 ? launch_stats_pass<double>(h, 1) : launch_stats_pass<float>(h, 1);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(s.data(), h->planes, sizeof(double) * 5 * Z, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    // statistics.c:54-60
+    for (int q = 0; q < 4; q++)
+        for (int z = 0; z < Z; z++) m[(size_t)q * Z + z] = s[(size_t)q * Z + z] / nxy;
+    HIPCHK(hipMemcpyAsync(h->means, m.data(), sizeof(double) * 4 * Z, hipMemcpyHostToDevice, h->stream));
+    rc = h->c.precision == DNSGPU_FP64 ? launch_stats_pass<double>(h, 2) : launch_stats_pass<float>(h, 2);
+    if (rc) return rc;
+    HIPCHK(hipMemcpyAsync(s.data(), h->planes, sizeof(double) * 5 * Z, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    memset(out, 0, sizeof(double) * 9 * zd);
+    for (int z = 0; z < Z; z++) {
+        const int k = z + 1;
+        out[0 * zd + k] = m[0 * (size_t)Z + z];
+        out[1 * zd + k] = m[1 * (size_t)Z + z];
+        out[2 * zd + k] = m[2 * (size_t)Z + z];
+        out[3 * zd + k] = sqrt(s[0 * (size_t)Z + z] / nxy);
+        out[4 * zd + k] = sqrt(s[1 * (size_t)Z + z] / nxy);
+        out[5 * zd + k] = sqrt(s[2 * (size_t)Z + z] / nxy);
+        out[6 * zd + k] = s[3 * (size_t)Z + z] / nxy;
+        out[7 * zd + k] = m[3 * (size_t)Z + z];
+        out[8 * zd + k] = sqrt(s[4 * (size_t)Z + z] / nxy);
+    }
+    return 0;
+}
+
+extern "C" int dnsgpu_fields(dnsgpu* h, int x0, int nx, double* vel, double* den)
+{
+    if (!h || !vel || !den) return set_err(DNSGPU_EARG, "null argument");
+    int rc = check_slab(h, x0, nx);
+    if (rc) return rc;
+    const size_t nn = (size_t)nx * h->g.Y * (h->g.Z + 2);
+    if ((rc = stage_reserve(h, nn * 4 * sizeof(double)))) return rc;
+    FieldArgs A;
+    A.g = h->g;
+    A.cur = h->buf[h->cur];
+    A.mbot = h->mbot;
+    A.mtop = h->mtop;
+    A.local = h->poststream;
+    A.x0 = x0;
+    A.nx = nx;
+    A.vel = h->stage;
+    A.den = h->stage + nn * 3;
+    int grid = (int)std::min<size_t>((nn + 255) / 256, 65536);
+    if (h->c.precision == DNSGPU_FP64)
+        hipLaunchKernelGGL(fields_kernel<double>, dim3(grid), dim3(256), 0, h->stream, A);
+    else
+        hipLaunchKernelGGL(fields_kernel<float>, dim3(grid), dim3(256), 0, h->stream, A);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(vel, A.vel, nn * 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(den, A.den, nn * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int dnsgpu_time_steps(dnsgpu* h, int nsteps, int with_stats, double* ms_kernels, double* ms_wall)
+{
+    if (!h || nsteps < 1) return set_err(DNSGPU_EARG, "bad argument");
+    std::vector<hipEvent_t> ev(2 * (size_t)nsteps);
+    for (auto& e : ev) HIPCHK(hipEventCreate(&e));
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    for (int s = 0; s < nsteps; s++) {
+        const void* src = h->buf[h->cur];
+        void* dst = h->buf[1 - h->cur];
+        StepArgs A = step_args(h, src, dst, h->nblk);
+        HIPCHK(hipEventRecord(ev[2 * s], h->stream));
+        const bool first = h->poststream != 0;
+        const bool st = with_stats != 0;
+#define LAUNCH_STEP(T)                                                                                         \
+    do {                                                                                                       \
+        if (first && st)                                                                                       \
+            hipLaunchKernelGGL((step_kernel<T, true, true>), dim3(h->nblk), dim3(h->block), 0, h->stream, A);  \
+        else if (first)                                                                                        \
+            hipLaunchKernelGGL((step_kernel<T, true, false>), dim3(h->nblk), dim3(h->block), 0, h->stream, A); \
+        else if (st)                                                                                           \
+            hipLaunchKernelGGL((step_kernel<T, false, true>), dim3(h->nblk), dim3(h->block), 0, h->stream, A); \
+        else                                                                                                   \
+            hipLaunchKernelGGL((step_kernel<T, false, false>), dim3(h->nblk), dim3(h->block), 0, h->stream,    \
+                               A);                                                                             \
+    } while (0)
+        if (h->c.precision == DNSGPU_FP64)
+            LAUNCH_STEP(double);
+        else
+            LAUNCH_STEP(float);
+#undef LAUNCH_STEP
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(ev[2 * s + 1], h->stream));
+        int rc = reduce_partials(h, st, h->nblk);
+        if (rc) return rc;
+        FinalizeArgs F = fin_args(h, st, nullptr, s, 0);
+        hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1024), 0, h->stream, F);
+        HIPCHK(hipGetLastError());
+        h->cur = 1 - h->cur;
+        h->poststream = 0;
+    }
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    HIPCHK(hipEventSynchronize(h->ev1));
+    double tk = 0.0;
+    for (int s = 0; s < nsteps; s++) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ev[2 * s], ev[2 * s + 1]));
+        tk += ms;
+    }
+    float wall = 0.f;
+    HIPCHK(hipEventElapsedTime(&wall, h->ev0, h->ev1));
+    for (auto& e : ev) (void)hipEventDestroy(e);
+    if (ms_kernels) *ms_kernels = tk;
+    if (ms_wall) *ms_wall = wall;
+    return 0;
+}

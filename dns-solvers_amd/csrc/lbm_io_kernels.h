@@ -1,0 +1,366 @@
+// lbm_io_kernels.h -- kernels that run at output cadence or at set-up:
+// checkpoint conversion (SoA device state <-> AoS lbf layout), the
+// statistics.c two-pass plane statistics, vel_print.c moments and the
+// initial-field generators.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lbm_kernels.h"
+
+namespace lbm {
+
+// ---------------------------------------------------------------- helpers
+
+template <typename T>
+__device__ __forceinline__ T from_f(double f, int a);
+template <>
+__device__ __forceinline__ double from_f<double>(double f, int) { return f; }
+template <>
+__device__ __forceinline__ float from_f<float>(double f, int a) { return (float)(f - WD[a]); }
+
+// Populations of the current post-stream state of node (col, z) as f (FP64).
+template <typename T>
+__device__ __forceinline__ void state_f(const T* src, long long P, const Column& c, int z, int Z, int Y,
+                                        const int8_t* mbot, const int8_t* mtop, bool local, double (&f)[Q])
+{
+    T v[Q];
+    if (local)
+        load_local19(src, P, c.n0 + (uint32_t)z, v);
+    else
+        pull19(src, P, c, z, Z, Y, mbot, mtop, v);
+#pragma unroll
+    for (int a = 0; a < Q; a++) f[a] = Store<T>::to_f(v[a], a);
+}
+
+// ------------------------------------------------------- checkpoint I/O
+
+struct AosArgs {
+    Geom g;
+    const void* cur;    // state buffer (post-stream if local, else post-collision)
+    const void* other;  // the "f" half when local (loaded previous post-collision)
+    void* cur_w;
+    void* other_w;
+    const int8_t* mbot;
+    const int8_t* mtop;
+    int local;
+    int x0, nx;
+    double* ftemp;  // staging [nx][Y][Z+2][19]
+    double* f;
+};
+
+// Device state -> lbf halves for x-planes [x0, x0+nx).  One thread per
+// (node, k) including the wall rows, which are written as zeros (the
+// reference never touches them; an init or checkpoint from the reference
+// holds zeros there).
+template <typename T>
+__global__ __launch_bounds__(256) void save_aos_kernel(AosArgs A)
+{
+    const int Z = A.g.Z, Y = A.g.Y, X = A.g.X;
+    const int Zd = Z + 2;
+    const long long total = (long long)A.nx * Y * Zd;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(i % Zd);
+        const long long cl = i / Zd;
+        const int y = (int)(cl % Y);
+        const int x = A.x0 + (int)(cl / Y);
+        double* ft = A.ftemp + i * Q;
+        double* fo = A.f + i * Q;
+        if (k == 0 || k == Zd - 1) {
+#pragma unroll
+            for (int a = 0; a < Q; a++) {
+                ft[a] = 0.0;
+                fo[a] = 0.0;
+            }
+            continue;
+        }
+        const int z = k - 1;
+        const Column c = make_column(x * Y + y, X, Y, Z);
+        double fs[Q];
+        state_f<T>((const T*)A.cur, A.g.P, c, z, Z, Y, A.mbot, A.mtop, A.local != 0, fs);
+        const T* ob = (const T*)(A.local ? A.other : A.cur);
+        const uint32_t n = c.n0 + (uint32_t)z;
+#pragma unroll
+        for (int a = 0; a < Q; a++) {
+            ft[a] = fs[a];
+            fo[a] = Store<T>::to_f(ob[(long long)a * A.g.P + n], a);
+        }
+    }
+}
+
+// lbf halves -> device: ftemp into cur_w, f into other_w (both local).
+template <typename T>
+__global__ __launch_bounds__(256) void load_aos_kernel(AosArgs A)
+{
+    const int Z = A.g.Z, Y = A.g.Y;
+    const long long total = (long long)A.nx * Y * Z;
+    T* cw = (T*)A.cur_w;
+    T* ow = (T*)A.other_w;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int z = (int)(i % Z);
+        const long long cl = i / Z;
+        const int y = (int)(cl % Y);
+        const int x = A.x0 + (int)(cl / Y);
+        const long long src = (cl * (Z + 2) + (z + 1)) * Q;
+        const long long n = ((long long)x * Y + y) * Z + z;
+#pragma unroll
+        for (int a = 0; a < Q; a++) {
+            cw[(long long)a * A.g.P + n] = from_f<T>(A.ftemp[src + a], a);
+            ow[(long long)a * A.g.P + n] = from_f<T>(A.f[src + a], a);
+        }
+    }
+}
+
+// ------------------------------------------------------- vel_print
+
+struct FieldArgs {
+    Geom g;
+    const void* cur;
+    const int8_t* mbot;
+    const int8_t* mtop;
+    int local;
+    int x0, nx;
+    double* vel;  // [nx][Y][Z+2][3]
+    double* den;  // [nx][Y][Z+2]
+};
+
+// vel_print.c:65-99: u = (sum e f)/dens, zeros where dens <= 0 (wall rows).
+template <typename T>
+__global__ __launch_bounds__(256) void fields_kernel(FieldArgs A)
+{
+    const int Z = A.g.Z, Y = A.g.Y, X = A.g.X;
+    const int Zd = Z + 2;
+    const long long total = (long long)A.nx * Y * Zd;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int k = (int)(i % Zd);
+        const long long cl = i / Zd;
+        const int y = (int)(cl % Y);
+        const int x = A.x0 + (int)(cl / Y);
+        double ux = 0., uy = 0., uz = 0., dens = 0.;
+        if (k >= 1 && k <= Z) {
+            const Column c = make_column(x * Y + y, X, Y, Z);
+            double f[Q];
+            state_f<T>((const T*)A.cur, A.g.P, c, k - 1, Z, Y, A.mbot, A.mtop, A.local != 0, f);
+            moments_fp64(f, ux, uy, uz, dens);
+        }
+        if (dens > 0.) {
+            ux /= dens;
+            uy /= dens;
+            uz /= dens;
+        } else {
+            ux = 0.;
+            uy = 0.;
+            uz = 0.;
+            dens = 0.;
+        }
+        A.vel[i * 3 + 0] = ux;
+        A.vel[i * 3 + 1] = uy;
+        A.vel[i * 3 + 2] = uz;
+        A.den[i] = dens;
+    }
+}
+
+// ------------------------------------------------------- statistics.c
+
+struct StatArgs {
+    Geom g;
+    const void* cur;
+    const int8_t* mbot;
+    const int8_t* mtop;
+    int local;
+    int pass;             // 1: sums of u,v,w,rho ; 2: sums of squared fluctuations
+    const double* means;  // [4][Z] UAVG,VAVG,WAVG,DAVG (pass 2)
+    double* part;         // [5][Z][nblk]
+    int nblk;
+};
+
+// Two-pass plane statistics (statistics.c:31-100); per-lane sums over the
+// wave's columns, then fixed-order block sums written as partials.
+template <typename T>
+__global__ __launch_bounds__(kWave* kMaxWaves) void stats_kernel(StatArgs A)
+{
+    __shared__ double red[kMaxWaves][5][kWave];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int nzc = A.g.nzc;
+    const int zc = wave % nzc;
+    const int slot = wave / nzc;
+    const int X = A.g.X, Y = A.g.Y, Z = A.g.Z;
+    const int z = zc * kWave + lane;
+    const bool zvalid = z < Z;
+    double acc[5] = {0., 0., 0., 0., 0.};
+    double mu = 0., mv = 0., mw = 0., md = 0.;
+    if (A.pass == 2 && zvalid) {
+        mu = A.means[z];
+        mv = A.means[Z + z];
+        mw = A.means[2 * Z + z];
+        md = A.means[3 * Z + z];
+    }
+    for (int grp = blockIdx.x; grp < A.g.ngroups; grp += gridDim.x) {
+        const int col = __builtin_amdgcn_readfirstlane(grp * A.g.cpb + slot);
+        if (col >= A.g.ncols || !zvalid) continue;
+        const Column c = make_column(col, X, Y, Z);
+        double f[Q];
+        state_f<T>((const T*)A.cur, A.g.P, c, z, Z, Y, A.mbot, A.mtop, A.local != 0, f);
+        double u, v, w, dens;
+        moments_fp64(f, u, v, w, dens);
+        if (A.pass == 1) {
+            acc[0] += (u / dens);
+            acc[1] += (v / dens);
+            acc[2] += (w / dens);
+            acc[3] += dens;
+        } else {
+            const double UF = (u / dens) - mu, VF = (v / dens) - mv, WF = (w / dens) - mw, DF = dens - md;
+            acc[0] += UF * UF;
+            acc[1] += VF * VF;
+            acc[2] += WF * WF;
+            acc[3] += UF * WF;
+            acc[4] += DF * DF;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 5; q++) red[wave][q][lane] = acc[q];
+    __syncthreads();
+    if (wave < nzc && z < Z) {
+#pragma unroll
+        for (int q = 0; q < 5; q++) {
+            double s = 0.0;
+            for (int sl = 0; sl < A.g.cpb; sl++) s += red[sl * nzc + zc][q][lane];
+            A.part[((long long)q * Z + z) * A.nblk + blockIdx.x] = s;
+        }
+    }
+}
+
+// ------------------------------------------------------- initial fields
+
+struct InitArgs {
+    Geom g;
+    void* cur;
+    void* other;
+    int kind;
+    double uc, amp;
+    unsigned long long seed;
+    int nmodes;
+    double nu;   // viscosity (turbulent profile)
+    double kappa_inv, Bplus;
+};
+
+__device__ __forceinline__ unsigned long long splitmix64(unsigned long long x)
+{
+    x += 0x9E3779B97F4A7C15ull;
+    x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
+    x = (x ^ (x >> 27)) * 0x94D049BB133111EBull;
+    return x ^ (x >> 31);
+}
+
+// Uniform in [-1, 1): 53 random bits, exact arithmetic.
+__device__ __forceinline__ double urand(unsigned long long seed, unsigned long long ctr)
+{
+    const unsigned long long h = splitmix64(seed ^ splitmix64(ctr));
+    return (double)(h >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
+}
+
+// feq(rho=1, u) with the reference's expression order (xcomputations.c:331).
+__device__ __forceinline__ void feq_fp64(double u0, double u1, double u2, double (&f)[Q])
+{
+    const double v = u0 * u0 + u1 * u1 + u2 * u2;
+    static_for<0, Q>([&](auto ac) {
+        constexpr int a = decltype(ac)::value;
+        const double u = edot<a>(u0, u1, u2);
+        if constexpr (a == 0)
+            f[a] = (WD[a] * 1.0) * (1. - 1.5 * v);
+        else
+            f[a] = (WD[a] * 1.0) * (((1. + 3. * u) + (4.5 * u) * u) - 1.5 * v);
+    });
+}
+
+// Seeded initial state, written as post-stream populations into cur (and
+// the same values into other, as init.c fills both f and ftemp).
+//  POISEUILLE: u_x = 4 uc y(H-y)/H^2, y = z + 1/2 (walls at 0 and H = Z),
+//              plus amp*uc*U[-1,1) in each component (counter = node*4+comp).
+//  TURBULENT : mean from the log law u+ = ln(y+)/kappa + B (viscous sublayer
+//              u+ = y+ below y+ = 11), plus the curl of a random vector
+//              potential built from Fourier modes kx, ky <= nmodes with a
+//              sin^2 wall-normal envelope (divergence-free, zero at walls).
+template <typename T>
+__global__ __launch_bounds__(256) void init_kernel(InitArgs A)
+{
+    const int X = A.g.X, Y = A.g.Y, Z = A.g.Z;
+    const long long total = (long long)X * Y * Z;
+    T* cw = (T*)A.cur;
+    T* ow = (T*)A.other;
+    for (long long n = blockIdx.x * (long long)blockDim.x + threadIdx.x; n < total;
+         n += (long long)gridDim.x * blockDim.x) {
+        const int z = (int)(n % Z);
+        const long long cl = n / Z;
+        const int y = (int)(cl % Y);
+        const int x = (int)(cl / Y);
+        double u0 = 0., u1 = 0., u2 = 0.;
+        if (A.kind == 1) {
+            const double yy = (double)z + 0.5;
+            const double H = (double)Z;
+            u0 = 4.0 * A.uc * yy * (H - yy) / (H * H);
+            const unsigned long long node = (unsigned long long)n;
+            u0 = u0 + A.amp * A.uc * urand(A.seed, node * 4ull + 0ull);
+            u1 = A.amp * A.uc * urand(A.seed, node * 4ull + 1ull);
+            u2 = A.amp * A.uc * urand(A.seed, node * 4ull + 2ull);
+        } else if (A.kind == 2) {
+            const double h = 0.5 * (double)Z;
+            const double yy = (double)z + 0.5;
+            const double yw = yy < h ? yy : (double)Z - yy;
+            const double yplus = yw * A.uc / A.nu;
+            const double uplus = yplus < 11.0 ? yplus : (log(yplus) * A.kappa_inv + A.Bplus);
+            u0 = A.uc * uplus;
+            const double ucl = A.uc * (log(h * A.uc / A.nu) * A.kappa_inv + A.Bplus);
+            // vector potential psi = (0, 0, 0) + sum over modes; u' = curl(psi)
+            // psi_x = env(z) * a1 cos(kx x + ky y + p1), psi_y = env(z) * a2 cos(...),
+            // with env = sin^2(pi (z+1/2)/Z); psi_z = 0 so that u'_z = d psi_y/dx - d psi_x/dy.
+            const double pz = 3.141592653589793 * yy / (double)Z;
+            const double env = sin(pz) * sin(pz);
+            const double denv = 2.0 * sin(pz) * cos(pz) * 3.141592653589793 / (double)Z;
+            double ux = 0., uy = 0., uz = 0.;
+            for (int i = 0; i <= A.nmodes; i++)
+                for (int j = -A.nmodes; j <= A.nmodes; j++) {
+                    if (i == 0 && j <= 0) continue;
+                    const unsigned long long m = (unsigned long long)((i * 64 + (j + 32)) & 0xFFFF);
+                    const double a1 = urand(A.seed, m * 8 + 0), a2 = urand(A.seed, m * 8 + 1);
+                    const double p1 = 3.141592653589793 * urand(A.seed, m * 8 + 2);
+                    const double p2 = 3.141592653589793 * urand(A.seed, m * 8 + 3);
+                    const double kx = 2.0 * 3.141592653589793 * i / (double)X;
+                    const double ky = 2.0 * 3.141592653589793 * j / (double)Y;
+                    const double ph1 = kx * x + ky * y + p1, ph2 = kx * x + ky * y + p2;
+                    const double sc = 1.0 / (double)(i * i + j * j);
+                    // psi_x = env a1 cos(ph1), psi_y = env a2 cos(ph2), psi_z = 0
+                    // u'_x = d psi_z/dy - d psi_y/dz = -denv a2 cos(ph2)
+                    // u'_y = d psi_x/dz - d psi_z/dx =  denv a1 cos(ph1)
+                    // u'_z = d psi_y/dx - d psi_x/dy = env(-a2 kx sin(ph2) + a1 ky sin(ph1))
+                    ux += sc * (-denv * a2 * cos(ph2));
+                    uy += sc * (denv * a1 * cos(ph1));
+                    uz += sc * env * (-a2 * kx * sin(ph2) + a1 * ky * sin(ph1));
+                }
+            // normalise the fluctuation amplitude to amp * centreline velocity
+            const double nrm = A.amp * ucl * (double)Z / 3.0;
+            u0 += nrm * ux;
+            u1 = nrm * uy;
+            u2 = nrm * uz;
+        }
+        double f[Q];
+        if (A.kind == 0) {
+#pragma unroll
+            for (int a = 0; a < Q; a++) f[a] = WD[a];
+        } else {
+            feq_fp64(u0, u1, u2, f);
+        }
+#pragma unroll
+        for (int a = 0; a < Q; a++) {
+            const T v = from_f<T>(f[a], a);
+            cw[(long long)a * A.g.P + n] = v;
+            ow[(long long)a * A.g.P + n] = v;
+        }
+    }
+}
+
+}  // namespace lbm

@@ -1,0 +1,220 @@
+"""ctypes binding of libdnsgpu.so (include/dnsgpu.h) for tests and benchmarks.
+
+The library is the product; this module only marshals arguments.  It never
+falls back to anything else: if libdnsgpu.so is missing or the GPU is absent
+the calls fail loudly.
+
+Array layouts follow the reference (DNS-solver/src/definitions.h:93):
+checkpoint halves are float64 [xdv][ydv][zdv+2][19].
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libdnsgpu.so")
+
+FP64, FP32, FP16 = 0, 1, 2
+CONST_Q, CONST_F = 0, 1
+INIT_REST, INIT_POISEUILLE, INIT_TURBULENT = 0, 1, 2
+ENAN = -2
+
+EXPORTS = (
+    "dnsgpu_create", "dnsgpu_destroy", "dnsgpu_last_error", "dnsgpu_set_walls",
+    "dnsgpu_set_solid_bands", "dnsgpu_init_rest", "dnsgpu_init_field", "dnsgpu_load_aos",
+    "dnsgpu_commit_load", "dnsgpu_save_aos", "dnsgpu_get_force", "dnsgpu_step",
+    "dnsgpu_statistics", "dnsgpu_fields", "dnsgpu_time_steps", "dnsgpu_device_bytes",
+)
+
+
+class DnsCase(ctypes.Structure):
+    _fields_ = [
+        ("xdv", ctypes.c_int), ("ydv", ctypes.c_int), ("zdv", ctypes.c_int),
+        ("Rebs", ctypes.c_double), ("ub", ctypes.c_double), ("CFL", ctypes.c_double),
+        ("TSTR", ctypes.c_int), ("forcing", ctypes.c_int), ("force", ctypes.c_double),
+        ("precision", ctypes.c_int), ("omega", ctypes.c_double),
+    ]
+
+
+LOG_FIELDS = ("t", "K", "flowrate", "massrate", "stup", "stdown", "stress", "pgrad", "force", "rho")
+
+
+class DnsStepLog(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in LOG_FIELDS]
+
+
+class DnsInit(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("uc", ctypes.c_double), ("amp", ctypes.c_double),
+                ("seed", ctypes.c_uint64), ("nmodes", ctypes.c_int)]
+
+
+class DnsGpuError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"dnsgpu error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", os.path.join(HERE, "csrc")], check=True)
+
+
+def load():
+    """Load libdnsgpu.so (no compute call: safe without a GPU)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} missing: run __graft_entry__.build() first")
+        L = ctypes.CDLL(LIB_PATH)
+        vp = ctypes.c_void_p
+        dp = ctypes.POINTER(ctypes.c_double)
+        i8 = ctypes.POINTER(ctypes.c_int8)
+        i16 = ctypes.POINTER(ctypes.c_int16)
+        L.dnsgpu_create.argtypes = [ctypes.POINTER(DnsCase), ctypes.POINTER(vp)]
+        L.dnsgpu_destroy.argtypes = [vp]
+        L.dnsgpu_destroy.restype = None
+        L.dnsgpu_last_error.restype = ctypes.c_char_p
+        L.dnsgpu_set_walls.argtypes = [vp, i8, i8]
+        L.dnsgpu_set_solid_bands.argtypes = [vp, i16, i16]
+        L.dnsgpu_init_rest.argtypes = [vp]
+        L.dnsgpu_init_field.argtypes = [vp, ctypes.POINTER(DnsInit)]
+        L.dnsgpu_load_aos.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.dnsgpu_commit_load.argtypes = [vp]
+        L.dnsgpu_save_aos.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.dnsgpu_get_force.argtypes = [vp, dp]
+        L.dnsgpu_step.argtypes = [vp, ctypes.c_long, ctypes.c_int, ctypes.POINTER(DnsStepLog)]
+        L.dnsgpu_statistics.argtypes = [vp, dp]
+        L.dnsgpu_fields.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.dnsgpu_time_steps.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.dnsgpu_device_bytes.argtypes = [vp]
+        L.dnsgpu_device_bytes.restype = ctypes.c_size_t
+        _lib = L
+    return _lib
+
+
+def _dp(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+
+
+def _chk(rc):
+    if rc != 0:
+        raise DnsGpuError(rc, load().dnsgpu_last_error().decode())
+
+
+def omega_of(Rebs, ub, zdv):
+    """dns-const-q-xycut.c:216-217."""
+    tau = (3. / Rebs) * ub * ((zdv + 2) - 2.) + 0.5
+    return 1. / tau
+
+
+class Channel:
+    """One channel case resident on the GPU (a dnsgpu handle)."""
+
+    def __init__(self, xdv, ydv, zdv, Rebs=100.0, ub=0.1, CFL=0.1, TSTR=0, forcing=CONST_Q,
+                 force=0.0, precision=FP64, omega=0.0):
+        self.X, self.Y, self.Z = xdv, ydv, zdv
+        self.case = DnsCase(xdv, ydv, zdv, Rebs, ub, CFL, TSTR, forcing, force, precision, omega)
+        self.h = ctypes.c_void_p()
+        _chk(load().dnsgpu_create(ctypes.byref(self.case), ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            load().dnsgpu_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def aos_shape(self, nx=None):
+        return (self.X if nx is None else nx, self.Y, self.Z + 2, 19)
+
+    def set_walls(self, bottom=None, top=None):
+        def p(m):
+            if m is None:
+                return None
+            m = np.ascontiguousarray(m, dtype=np.int8)
+            assert m.shape == (self.X, self.Y)
+            return m
+        b, t = p(bottom), p(top)
+        self._walls = (b, t)
+        _chk(load().dnsgpu_set_walls(
+            self.h,
+            b.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)) if b is not None else None,
+            t.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)) if t is not None else None))
+
+    def init_rest(self):
+        _chk(load().dnsgpu_init_rest(self.h))
+
+    def init_field(self, kind, uc=0.0, amp=0.0, seed=1, nmodes=4):
+        spec = DnsInit(kind, uc, amp, seed, nmodes)
+        _chk(load().dnsgpu_init_field(self.h, ctypes.byref(spec)))
+
+    def load_aos(self, ftemp, f, slab=None):
+        ftemp = np.ascontiguousarray(ftemp, dtype=np.float64)
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        assert ftemp.shape == self.aos_shape() and f.shape == self.aos_shape()
+        slab = slab or self.X
+        for x0 in range(0, self.X, slab):
+            nx = min(slab, self.X - x0)
+            _chk(load().dnsgpu_load_aos(self.h, x0, nx, _dp(ftemp[x0:x0 + nx]), _dp(f[x0:x0 + nx])))
+        _chk(load().dnsgpu_commit_load(self.h))
+
+    def save_aos(self, slab=None):
+        ft = np.zeros(self.aos_shape())
+        f = np.zeros(self.aos_shape())
+        slab = slab or self.X
+        for x0 in range(0, self.X, slab):
+            nx = min(slab, self.X - x0)
+            a = np.zeros(self.aos_shape(nx))
+            b = np.zeros(self.aos_shape(nx))
+            _chk(load().dnsgpu_save_aos(self.h, x0, nx, _dp(a), _dp(b)))
+            ft[x0:x0 + nx] = a
+            f[x0:x0 + nx] = b
+        return ft, f
+
+    def force(self):
+        g = ctypes.c_double()
+        _chk(load().dnsgpu_get_force(self.h, ctypes.byref(g)))
+        return g.value
+
+    def step(self, n, cntr0=0, logs=True):
+        if logs:
+            rows = (DnsStepLog * max(n, 1))()
+            _chk(load().dnsgpu_step(self.h, cntr0, n, rows))
+            return {k: np.array([getattr(rows[i], k) for i in range(n)]) for k in LOG_FIELDS}
+        _chk(load().dnsgpu_step(self.h, cntr0, n, None))
+        return None
+
+    def statistics(self):
+        out = np.zeros((9, self.Z + 2))
+        _chk(load().dnsgpu_statistics(self.h, _dp(out)))
+        return out
+
+    def fields(self):
+        vel = np.zeros((self.X, self.Y, self.Z + 2, 3))
+        den = np.zeros((self.X, self.Y, self.Z + 2))
+        _chk(load().dnsgpu_fields(self.h, 0, self.X, _dp(vel), _dp(den)))
+        return vel, den
+
+    def time_steps(self, n, stats=False):
+        tk = ctypes.c_double()
+        tw = ctypes.c_double()
+        _chk(load().dnsgpu_time_steps(self.h, n, 1 if stats else 0, ctypes.byref(tk), ctypes.byref(tw)))
+        return tk.value, tw.value
+
+    def device_bytes(self):
+        return load().dnsgpu_device_bytes(self.h)

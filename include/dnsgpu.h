@@ -1,0 +1,177 @@
+/*
+ * dnsgpu.h -- C ABI of the MI355X (gfx950) D3Q19 BGK channel-flow step.
+ *
+ * The library replaces the per-rank update loop of DNS-solver
+ * (DNS-solver/src/dns-const-q-xycut.c:485-519): the three physics calls
+ *
+ *   POINTER encalc(int xd,int yd,int zd,POINTER V,DP Gx,int TM,long cntr,
+ *                  DP ubulk,MPI_Comm)                       fun_defs.h:5
+ *   POINTER wshearstr(int xd,int yd,int zd,POINTER V,DP Gx,DP tau,int TM,
+ *                     long cntr,MPI_Comm)                   fun_defs.h:9
+ *   POINTER xcomputations(int xd,int yd,int zd,POINTER V,PDATA pd,DP Gx,
+ *                         DP Gy,DP Gz,DP rhozero,DP tau,DP ubulk)
+ *                                                           fun_defs.h:6
+ *
+ * and the cadence outputs statistics() (fun_defs.h:10), vel_print()
+ * (fun_defs.h:11), output_twoD() (fun_defs.h:7) and the lbf/force
+ * checkpoint (dns-const-q-xycut.c:405-435,491-514).  Where the reference
+ * passes a POINTER struct by value and keeps the populations in host
+ * memory, this ABI keeps the state on the GPU behind an opaque handle:
+ * host arrays cross the boundary only at output cadence, in the
+ * reference's own file layouts.
+ *
+ * Conventions
+ *  - Axes and names as in the reference: x streamwise (xdv), y spanwise
+ *    (ydv), z wall-normal (zdv fluid rows; the wall rows k=0 and k=zdv+1
+ *    are implicit), definitions.h:30-34.
+ *  - Host population arrays use the checkpoint layout
+ *    [x][y][k=0..zdv+1][19] float64 (definitions.h:93, ghost planes
+ *    excluded as in dns-const-q-xycut.c:144-155).
+ *  - Every call returns 0 on success or a negative DNSGPU_E* code;
+ *    dnsgpu_last_error() gives the message.  A NaN in the wall stress
+ *    returns DNSGPU_ENAN (the reference aborts, wshearstr.c:126-130).
+ *  - One host thread per handle; all work is ordered on the handle's
+ *    HIP stream.  No entry point calls the HIP runtime's device-wide
+ *    synchronisation.
+ */
+#ifndef DNSGPU_H
+#define DNSGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DNSGPU_OK 0
+#define DNSGPU_EHIP (-1)   /* HIP runtime error */
+#define DNSGPU_ENAN (-2)   /* NaN in the wall stress (wshearstr.c:126-130) */
+#define DNSGPU_EARG (-3)   /* bad size / argument */
+#define DNSGPU_ENOMEM (-4) /* device allocation failed */
+
+enum dnsgpu_precision {
+    DNSGPU_FP64 = 0,  /* populations f in FP64, reference arithmetic bit for bit */
+    DNSGPU_FP32 = 1,  /* shifted populations g = f - w in FP32, FP32 collision */
+    DNSGPU_FP16 = 2   /* shifted populations in FP16 storage, FP32 collision */
+};
+
+enum dnsgpu_forcing {
+    DNSGPU_CONST_Q = 0, /* force = rho_mean*ub - jx_mean every step (encalc.c:121) */
+    DNSGPU_CONST_F = 1  /* constant body force */
+};
+
+/* Case inputs: the macros of definitions.h:30-37 plus the build's own keys. */
+typedef struct dns_case {
+    int xdv, ydv, zdv;  /* grid (definitions.h:32-34) */
+    double Rebs;        /* bulk Reynolds number (definitions.h:30) */
+    double ub;          /* bulk velocity (definitions.h:31) */
+    double CFL;         /* output cadence utime = (int)(0.5*zdv/CFL) (definitions.h:37) */
+    int TSTR;           /* time label of the first step (definitions.h:35) */
+    int forcing;        /* enum dnsgpu_forcing */
+    double force;       /* body force for DNSGPU_CONST_F */
+    int precision;      /* enum dnsgpu_precision */
+    double omega;       /* 1/tau; 0 => (3/Rebs)*ub*zdv+0.5 inverted (dns-const-q-xycut.c:216-217) */
+} dns_case;
+
+/* One row of the seven per-step logs, computed from the state before the
+ * step's collision: encalc.c:125-138 and wshearstr.c:133-154. */
+typedef struct dns_steplog {
+    double t;        /* TM + dt*cntr (encalc.c:128) */
+    double K;        /* energy.txt */
+    double flowrate; /* flowrate.txt */
+    double massrate; /* massrate.txt */
+    double stup;     /* str-up.txt */
+    double stdown;   /* str-do.txt */
+    double stress;   /* w-str.txt */
+    double pgrad;    /* p-grad.txt = force * zdv/2 */
+    double force;    /* the V.force this step's collision used */
+    double rho;      /* V.rho, mean density (encalc.c:79) */
+} dns_steplog;
+
+/* Initial fields generated on the device. */
+enum dnsgpu_init_kind {
+    DNSGPU_INIT_REST = 0,       /* f = w everywhere (init.c:14-20) */
+    DNSGPU_INIT_POISEUILLE = 1, /* feq(1, u): u_x = 4 uc y(H-y)/H^2 + seeded noise */
+    DNSGPU_INIT_TURBULENT = 2   /* feq(1, u): log-law mean + divergence-free Fourier modes */
+};
+
+typedef struct dns_init {
+    int kind;      /* enum dnsgpu_init_kind */
+    double uc;     /* centreline velocity (POISEUILLE) / friction velocity u_tau (TURBULENT) */
+    double amp;    /* noise amplitude as a fraction of uc (POISEUILLE) or of the centreline (TURBULENT) */
+    uint64_t seed; /* counter-based RNG seed */
+    int nmodes;    /* TURBULENT: Fourier modes per periodic direction (kx, ky <= nmodes) */
+} dns_init;
+
+typedef struct dnsgpu dnsgpu;
+
+/* Allocate the device state for a case.  Replaces the calloc block of
+ * dns-const-q-xycut.c:239-356 (no halo buffers: the periodic x/y wrap is
+ * done inside the kernel). */
+int dnsgpu_create(const dns_case *c, dnsgpu **out);
+void dnsgpu_destroy(dnsgpu *h);
+const char *dnsgpu_last_error(void);
+
+/* Wall masks [xdv][ydv] for the bottom (k=0) and top (k=zdv+1) rows with
+ * the node codes of solid_init.c:3-8: -1 no-slip, -2 slip.  NULL means all
+ * no-slip.  Replaces solid_init()/slip_ridges_init()/slip_posts_init()
+ * (fun_defs.h:1-4) for the wall rows. */
+int dnsgpu_set_walls(dnsgpu *h, const int8_t *bottom, const int8_t *top);
+
+/* Solid bands inside the fluid rows (riblets, posts with side walls): per
+ * column the number of solid fluid rows above the bottom wall and below the
+ * top wall, [xdv][ydv] each, NULL = 0.  No reference counterpart (the
+ * reference only has solids in the wall rows); rule in the step kernel. */
+int dnsgpu_set_solid_bands(dnsgpu *h, const int16_t *bottom, const int16_t *top);
+
+/* Rest start, init.c:3-20 (the '-s' branch, dns-const-q-xycut.c:385-398). */
+int dnsgpu_init_rest(dnsgpu *h);
+
+/* Seeded initial fields (the build's generator; see dns_init). */
+int dnsgpu_init_field(dnsgpu *h, const dns_init *spec);
+
+/* Restart ('-p' branch, dns-const-q-xycut.c:399-454): upload x-planes
+ * [x0, x0+nx) of both checkpoint halves.  ftemp is the state (post-stream
+ * populations), f the previous post-collision populations, each
+ * [nx][ydv][zdv+2][19] float64.  After the last slab call
+ * dnsgpu_commit_load() derives the force of the first step. */
+int dnsgpu_load_aos(dnsgpu *h, int x0, int nx, const double *ftemp, const double *f);
+int dnsgpu_commit_load(dnsgpu *h);
+
+/* Checkpoint (dns-const-q-xycut.c:491-503): download x-planes [x0, x0+nx)
+ * of both halves in the same layout.  FP32/FP16 states are written as
+ * float64 w + g. */
+int dnsgpu_save_aos(dnsgpu *h, int x0, int nx, double *ftemp, double *f);
+
+/* Current V.force (the force the next step will use); force.NNNN. */
+int dnsgpu_get_force(dnsgpu *h, double *force);
+
+/* Advance nsteps time steps: for each step, encalc + wshearstr + one fused
+ * collide-stream (dns-const-q-xycut.c:487-488,518).  cntr0 is the loop
+ * counter `ss` of the first step (time label TM + dt*ss).  rows, if not
+ * NULL, receives nsteps log rows; with rows == NULL the per-step plane
+ * statistics are skipped (the force budget is always kept). */
+int dnsgpu_step(dnsgpu *h, long cntr0, int nsteps, dns_steplog *rows);
+
+/* statistics.c:3-184 of the current state: out[9][zdv+2] in the order
+ * UAVG,VAVG,WAVG,UTIN,VTIN,WTIN,REYS,DAVG,DTIN (statistics.c:139-147). */
+int dnsgpu_statistics(dnsgpu *h, double *out);
+
+/* vel_print.c:65-99 of the current state for x-planes [x0,x0+nx):
+ * vel[nx][ydv][zdv+2][3], den[nx][ydv][zdv+2] (zeros on wall rows). */
+int dnsgpu_fields(dnsgpu *h, int x0, int nx, double *vel, double *den);
+
+/* Kernel-only timing helpers for benchmarks: time nsteps steps on the
+ * handle's stream with HIP events; ms_total receives the elapsed time of
+ * the step kernels alone (sum over steps) and ms_wall the wall time of the
+ * whole sequence. */
+int dnsgpu_time_steps(dnsgpu *h, int nsteps, int with_stats, double *ms_step_kernels, double *ms_wall);
+
+/* Device bytes held by the handle. */
+size_t dnsgpu_device_bytes(const dnsgpu *h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -37,3 +37,86 @@ def poiseuille_field(X, Y, Z, uc, noise=0.05, seed=1, solid=None):
     if solid is not None:
         f[solid] = 0
     return np.ascontiguousarray(f)
+
+
+# --------------------------------------------------------------------------
+# Bit-exact replica of the library's DNSGPU_INIT_POISEUILLE generator
+# (dns-solvers_amd/csrc/lbm_io_kernels.h, init_kernel): counter-based
+# splitmix64 noise and the feq expression order of xcomputations.c:331.
+# Every step is an IEEE operation numpy performs identically.
+
+_M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
+
+
+def _splitmix64(x):
+    x = (x + np.uint64(0x9E3779B97F4A7C15))
+    x = (x ^ (x >> np.uint64(30))) * np.uint64(0xBF58476D1CE4E5B9)
+    x = (x ^ (x >> np.uint64(27))) * np.uint64(0x94D049BB133111EB)
+    return x ^ (x >> np.uint64(31))
+
+
+def _urand(seed, ctr):
+    with np.errstate(over="ignore"):
+        h = _splitmix64(np.uint64(seed) ^ _splitmix64(ctr.astype(np.uint64)))
+    return (h >> np.uint64(11)).astype(np.float64) * (1.0 / 9007199254740992.0) * 2.0 - 1.0
+
+
+def _edot(a, u0, u1, u2):
+    comps = [(int(E[a, 0]), u0), (int(E[a, 1]), u1), (int(E[a, 2]), u2)]
+    s = None
+    for e, c in comps:
+        if e == 0:
+            continue
+        if s is None:
+            s = c if e > 0 else -c
+        else:
+            s = (s + c) if e > 0 else (s - c)
+    return np.zeros_like(u0) if s is None else s
+
+
+def feq_ref_order(u0, u1, u2):
+    """feq(rho=1, u) in the reference's association order -> [..., 19]."""
+    v = u0 * u0 + u1 * u1 + u2 * u2
+    out = np.empty(u0.shape + (19,))
+    for a in range(19):
+        u = _edot(a, u0, u1, u2)
+        if a == 0:
+            out[..., a] = (W[a] * 1.0) * (1. - 1.5 * v)
+        else:
+            out[..., a] = (W[a] * 1.0) * (((1. + 3. * u) + (4.5 * u) * u) - 1.5 * v)
+    return out
+
+
+def device_poiseuille(X, Y, Z, uc, amp, seed):
+    """AoS [X][Y][Z+2][19] of the library's POISEUILLE initial state (FP64)."""
+    n = np.arange(X * Y * Z, dtype=np.uint64).reshape(X, Y, Z)
+    z = (np.arange(Z, dtype=np.float64) + 0.5)[None, None, :]
+    H = float(Z)
+    u0 = 4.0 * uc * z * (H - z) / (H * H)
+    u0 = np.broadcast_to(u0, (X, Y, Z)).copy()
+    with np.errstate(over="ignore"):
+        c4 = n * np.uint64(4)
+    u0 = u0 + amp * uc * _urand(seed, c4)
+    u1 = amp * uc * _urand(seed, c4 + np.uint64(1))
+    u2 = amp * uc * _urand(seed, c4 + np.uint64(2))
+    f = np.zeros((X, Y, Z + 2, 19))
+    f[:, :, 1:Z + 1, :] = feq_ref_order(u0, u1, u2)
+    return f
+
+
+def posts_mask(X, Y):
+    """Wall codes of slip_posts_init.c:11-41 on the periodic domain: 2x2
+    slip squares with period 4 in x and y, starting at index 1."""
+    i = np.arange(X)[:, None] % 4
+    j = np.arange(Y)[None, :] % 4
+    m = np.full((X, Y), -1, dtype=np.int8)
+    m[((i == 1) | (i == 2)) & ((j == 1) | (j == 2))] = -2
+    return m
+
+
+def stripes_mask(X, Y, period, slip_width, offset=0, along="x"):
+    """Streamwise-invariant slip stripes (grooves): slip where
+    (j - offset) mod period < slip_width (j spanwise)."""
+    j = (np.arange(Y) - offset) % period
+    row = np.where(j < slip_width, -2, -1).astype(np.int8)
+    return np.ascontiguousarray(np.broadcast_to(row[None, :], (X, Y)))

@@ -1,0 +1,166 @@
+"""Parity of the HIP library against the oracle (CPU restatement, pinned
+bitwise to the reference binary) on seeded inputs.
+
+Bars (stated per test):
+  * constant-F FP64: populations bitwise equal to the oracle after N steps
+    (same arithmetic in the same order; no reduction feeds the state).
+  * constant-Q FP64: populations within 1e-12 relative (the force is a
+    global sum; the GPU sums in a fixed tree order, the reference
+    sequentially); log rows within 1e-10 relative.
+  * FP32-shifted: error against FP64 stated per quantity.
+"""
+import numpy as np
+import pytest
+
+import dnsgpu
+import fields
+import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+def make_pair(X, Y, Z, forcing, F=0.0, mask2d=None, precision=dnsgpu.FP64, Rebs=60.0, ub=0.05, CFL=0.25,
+              TSTR=100):
+    ch = dnsgpu.Channel(X, Y, Z, Rebs=Rebs, ub=ub, CFL=CFL, TSTR=TSTR, forcing=forcing, force=F,
+                        precision=precision)
+    bot = top = None
+    if mask2d is not None:
+        bot, top = mask2d
+        ch.set_walls(bot, top)
+    om = pyoracle.omega(Rebs, ub, Z)
+    o = pyoracle.Oracle(X, Y, Z, om, ub, CFL, TM=TSTR, forcing=1 if forcing == dnsgpu.CONST_F else 0, F=F,
+                        mask=pyoracle.wall_mask(X, Y, Z, bot, top))
+    return ch, o
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def seeded(ch, uc=0.06, amp=0.05, seed=7):
+    ch.init_field(dnsgpu.INIT_POISEUILLE, uc=uc, amp=amp, seed=seed)
+    return fields.device_poiseuille(ch.X, ch.Y, ch.Z, uc, amp, seed)
+
+
+def test_init_field_matches_replica_bitwise():
+    print("case: init POISEUILLE 12x8x70", flush=True)
+    ch = dnsgpu.Channel(12, 8, 70)
+    f0 = seeded(ch)
+    ft, f = ch.save_aos()
+    assert np.array_equal(ft, f0)
+    assert np.array_equal(f, f0)
+
+
+def test_save_load_roundtrip_bitwise():
+    X, Y, Z = 8, 6, 12
+    rng = np.random.default_rng(3)
+    a = rng.uniform(0.01, 0.4, size=(X, Y, Z + 2, 19))
+    b = rng.uniform(0.01, 0.4, size=(X, Y, Z + 2, 19))
+    a[:, :, 0] = a[:, :, -1] = 0
+    b[:, :, 0] = b[:, :, -1] = 0
+    ch = dnsgpu.Channel(X, Y, Z)
+    ch.load_aos(a, b, slab=3)
+    ft, f = ch.save_aos(slab=5)
+    assert np.array_equal(ft, a) and np.array_equal(f, b)
+
+
+@pytest.mark.parametrize("shape,walls", [
+    ((16, 8, 10), "smooth"),
+    ((16, 8, 10), "posts"),
+    ((16, 12, 70), "stripes"),
+    ((8, 8, 130), "posts"),
+])
+def test_const_force_fp64_bitwise(shape, walls):
+    X, Y, Z = shape
+    print(f"case: const-F FP64 {shape} {walls}", flush=True)
+    mask = None
+    if walls == "posts":
+        m = fields.posts_mask(X, Y)
+        mask = (m, m)
+    elif walls == "stripes":
+        mask = (fields.stripes_mask(X, Y, 6, 3), fields.stripes_mask(X, Y, 4, 1, offset=2))
+    ch, o = make_pair(X, Y, Z, dnsgpu.CONST_F, F=2e-5, mask2d=mask)
+    f0 = seeded(ch)
+    n = 60
+    logs = ch.step(n, cntr0=0)
+    ft, f = ch.save_aos()
+    oft, of = f0.copy(), f0.copy()
+    ologs = o.run(oft, of, n)
+    assert np.array_equal(ft, oft), rel(ft, oft)
+    assert np.array_equal(f[:, :, 1:Z + 1], of[:, :, 1:Z + 1])
+    for k in ("K", "flowrate", "massrate", "stup", "stdown", "stress", "pgrad", "rho"):
+        assert np.allclose(logs[k], ologs[k], rtol=1e-10, atol=1e-300), (k, logs[k][:3], ologs[k][:3])
+    assert np.array_equal(logs["t"], ologs["t"])
+
+
+@pytest.mark.parametrize("walls", ["smooth", "posts"])
+def test_const_q_fp64_within_1e12(walls):
+    X, Y, Z = 16, 8, 10
+    print(f"case: const-Q FP64 {walls}", flush=True)
+    mask = None
+    if walls == "posts":
+        m = fields.posts_mask(X, Y)
+        mask = (m, m)
+    ch, o = make_pair(X, Y, Z, dnsgpu.CONST_Q, mask2d=mask)
+    f0 = seeded(ch)
+    n = 200
+    logs = ch.step(n)
+    ft, f = ch.save_aos()
+    oft, of = f0.copy(), f0.copy()
+    ologs = o.run(oft, of, n)
+    assert rel(ft, oft) <= 1e-12
+    for k in ("K", "flowrate", "massrate", "stup", "stdown", "stress", "pgrad", "force"):
+        scale = np.abs(ologs[k]).max()
+        assert np.abs(logs[k] - ologs[k]).max() <= 1e-10 * scale, k
+
+
+def test_statistics_and_fields_match_oracle():
+    X, Y, Z = 16, 8, 10
+    print("case: statistics/fields", flush=True)
+    m = fields.posts_mask(X, Y)
+    ch, o = make_pair(X, Y, Z, dnsgpu.CONST_Q, mask2d=(m, m))
+    seeded(ch)
+    ch.step(37, logs=False)
+    ft, _ = ch.save_aos()
+    st = ch.statistics()
+    ost = o.statistics(ft)
+    assert np.allclose(st, ost, rtol=1e-10, atol=1e-15)
+    vel, den = ch.fields()
+    ovel, oden = o.moments(ft)
+    assert np.array_equal(vel, ovel) and np.array_equal(den, oden)
+
+
+def test_restart_equals_uninterrupted():
+    X, Y, Z = 16, 8, 10
+    print("case: restart", flush=True)
+    ch, _ = make_pair(X, Y, Z, dnsgpu.CONST_F, F=1e-5)
+    seeded(ch)
+    ch.step(20, logs=False)
+    a, b = ch.save_aos()
+    ch.step(30, logs=False)
+    end1 = ch.save_aos()[0]
+    ch2, _ = make_pair(X, Y, Z, dnsgpu.CONST_F, F=1e-5)
+    ch2.load_aos(a, b)
+    ch2.step(30, logs=False)
+    end2 = ch2.save_aos()[0]
+    assert np.array_equal(end1, end2)
+
+
+def test_fp32_shifted_against_fp64():
+    X, Y, Z = 16, 8, 24
+    print("case: fp32 vs fp64", flush=True)
+    m = fields.stripes_mask(X, Y, 4, 2, offset=1)
+    ch64, _ = make_pair(X, Y, Z, dnsgpu.CONST_Q, mask2d=(m, m))
+    ch32, _ = make_pair(X, Y, Z, dnsgpu.CONST_Q, mask2d=(m, m), precision=dnsgpu.FP32)
+    seeded(ch64)
+    seeded(ch32)
+    l64 = ch64.step(300)
+    l32 = ch32.step(300)
+    s64 = ch64.statistics()
+    s32 = ch32.statistics()
+    # mean profile within 1e-4 relative, populations within 1e-5 of f
+    assert np.abs(s32[0] - s64[0]).max() <= 1e-4 * np.abs(s64[0]).max()
+    f64 = ch64.save_aos()[0]
+    f32 = ch32.save_aos()[0]
+    assert np.abs(f32 - f64).max() <= 1e-5
+    assert np.abs(l32["stress"] - l64["stress"]).max() <= 1e-3 * np.abs(l64["stress"]).max()
