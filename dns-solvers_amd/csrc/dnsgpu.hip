@@ -50,6 +50,7 @@ struct dnsgpu {
     int poststream;  // 1: buf[cur] holds post-stream populations (after load/init)
     int8_t* mbot;
     int8_t* mtop;
+    uint32_t* wword;
     int16_t* hbot;
     int16_t* htop;
     double* part;     // [kNQ][Z][nblk]
@@ -117,6 +118,7 @@ static StepArgs step_args(dnsgpu* h, const void* src, void* dst, int nblk)
     A.dst = dst;
     A.mbot = h->mbot;
     A.mtop = h->mtop;
+    A.wword = h->wword;
     A.omega = h->omega;
     A.G = &h->st->G;
     A.shift = h->shift;
@@ -221,6 +223,15 @@ static int refresh_force(dnsgpu* h)
     return launch_moments<float>(h);
 }
 
+static int update_wall_words(dnsgpu* h)
+{
+    const int xy = h->g.X * h->g.Y;
+    hipLaunchKernelGGL(wall_word_kernel, dim3((xy + 255) / 256), dim3(256), 0, h->stream, h->mbot, h->mtop, h->g.X,
+                       h->g.Y, h->wword);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
 extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
 {
     if (!c || !out) return set_err(DNSGPU_EARG, "null argument");
@@ -288,6 +299,7 @@ extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
         {&h->buf[1], popbytes},
         {(void**)&h->mbot, xy},
         {(void**)&h->mtop, xy},
+        {(void**)&h->wword, xy * 4},
         {(void**)&h->hbot, xy * 2},
         {(void**)&h->htop, xy * 2},
         {(void**)&h->part, sizeof(double) * kNQ * g.Z * (size_t)h->nblk},
@@ -311,6 +323,7 @@ extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
         hipMemsetAsync(h->st, 0, sizeof(DevState), h->stream) != hipSuccess ||
         hipMemsetAsync(h->shift, 0, sizeof(double) * 3 * g.Z, h->stream) != hipSuccess)
         return fail(set_err(DNSGPU_EHIP, "initial memset failed"));
+    if (update_wall_words(h)) return fail(DNSGPU_EHIP);
     // the pop buffers start at rest
     h->cur = 0;
     h->poststream = 1;
@@ -325,7 +338,7 @@ extern "C" void dnsgpu_destroy(dnsgpu* h)
 {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    void* ptrs[] = {h->buf[0], h->buf[1], h->mbot, h->mtop, h->hbot,   h->htop, h->part,
+    void* ptrs[] = {h->buf[0], h->buf[1], h->mbot, h->mtop, h->wword, h->hbot, h->htop, h->part,
                     h->gpart,  h->planes, h->shift, h->means, h->st, h->ring, h->stage};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -353,9 +366,9 @@ extern "C" int dnsgpu_set_walls(dnsgpu* h, const int8_t* bottom, const int8_t* t
             HIPCHK(hipMemsetAsync(dst[w], -1, xy, h->stream));
         }
     }
+    int rc = update_wall_words(h);
+    if (rc) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
-    // the first step's force depends on the state only; the budget of later
-    // steps reads the masks directly.
     return 0;
 }
 

@@ -76,7 +76,7 @@ __global__ __launch_bounds__(256) void save_aos_kernel(AosArgs A)
             continue;
         }
         const int z = k - 1;
-        const Column c = make_column(x * Y + y, X, Y, Z);
+        const Column c = make_column_lane(x * Y + y, X, Y, Z);
         double fs[Q];
         state_f<T>((const T*)A.cur, A.g.P, c, z, Z, Y, A.mbot, A.mtop, A.local != 0, fs);
         const T* ob = (const T*)(A.local ? A.other : A.cur);
@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void fields_kernel(FieldArgs A)
         const int x = A.x0 + (int)(cl / Y);
         double ux = 0., uy = 0., uz = 0., dens = 0.;
         if (k >= 1 && k <= Z) {
-            const Column c = make_column(x * Y + y, X, Y, Z);
+            const Column c = make_column_lane(x * Y + y, X, Y, Z);
             double f[Q];
             state_f<T>((const T*)A.cur, A.g.P, c, k - 1, Z, Y, A.mbot, A.mtop, A.local != 0, f);
             moments_fp64(f, ux, uy, uz, dens);

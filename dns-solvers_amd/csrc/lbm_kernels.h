@@ -55,6 +55,7 @@ struct StepArgs {
     void* dst;
     const int8_t* mbot;  // [X*Y] bottom wall codes (-1 / -2)
     const int8_t* mtop;  // [X*Y] top wall codes
+    const uint32_t* wword;  // [X*Y] per-column wall words
     double omega;
     const double* G;      // force used by this step (device scalar)
     const double* shift;  // [3][Z] plane-mean shifts for the one-pass variances
@@ -79,30 +80,78 @@ struct Store<float> {
 __device__ __forceinline__ int wrap_dec(int v, int n) { return v == 0 ? n - 1 : v - 1; }
 __device__ __forceinline__ int wrap_inc(int v, int n) { return v == n - 1 ? 0 : v + 1; }
 
-// Column geometry of one work item: the nine upstream column bases.
+// Column geometry of one work item.  All fields are wave-uniform; they are
+// forced into SGPRs with readfirstlane so every address below is a scalar
+// base plus the lane's z offset.
 struct Column {
     int x, y, xm, xp, ym, yp;
-    uint32_t cb[3][3];  // cb[ex+1][ey+1] = base of column (x - ex, y - ey)
-    uint32_t n0;        // own column base
+    int Y, Z;
+    uint32_t n0;  // own column base (x*Y + y)*Z
 };
 
-__device__ __forceinline__ Column make_column(int col, int X, int Y, int Z)
+// Per-thread column (the thread's own column; no uniformity assumed).
+__device__ __forceinline__ Column make_column_lane(int col, int X, int Y, int Z)
 {
     Column c;
     c.x = col / Y;
     c.y = col - c.x * Y;
-    c.xm = wrap_dec(c.x, X);
-    c.xp = wrap_inc(c.x, X);
-    c.ym = wrap_dec(c.y, Y);
-    c.yp = wrap_inc(c.y, Y);
-    const int sx[3] = {c.xp, c.x, c.xm};  // ex = -1, 0, +1 -> x - ex
-    const int sy[3] = {c.yp, c.y, c.ym};
-#pragma unroll
-    for (int i = 0; i < 3; i++)
-#pragma unroll
-        for (int j = 0; j < 3; j++) c.cb[i][j] = (uint32_t)((sx[i] * Y + sy[j]) * Z);
-    c.n0 = c.cb[1][1];
+    c.xm = c.x == 0 ? X - 1 : c.x - 1;
+    c.xp = c.x == X - 1 ? 0 : c.x + 1;
+    c.ym = c.y == 0 ? Y - 1 : c.y - 1;
+    c.yp = c.y == Y - 1 ? 0 : c.y + 1;
+    c.Y = Y;
+    c.Z = Z;
+    c.n0 = (uint32_t)((c.x * Y + c.y) * Z);
     return c;
+}
+
+// Wave-uniform column (col must be the same in every lane of the wave).
+__device__ __forceinline__ Column make_column(int col, int X, int Y, int Z)
+{
+    Column c;
+    c.x = __builtin_amdgcn_readfirstlane(col / Y);
+    c.y = __builtin_amdgcn_readfirstlane(col - c.x * Y);
+    c.xm = c.x == 0 ? X - 1 : c.x - 1;
+    c.xp = c.x == X - 1 ? 0 : c.x + 1;
+    c.ym = c.y == 0 ? Y - 1 : c.y - 1;
+    c.yp = c.y == Y - 1 ? 0 : c.y + 1;
+    c.Y = Y;
+    c.Z = Z;
+    c.n0 = (uint32_t)((c.x * Y + c.y) * Z);
+    return c;
+}
+
+// Base of the column (x - ex, y - ey) that direction (ex, ey) pulls from.
+template <int ex, int ey>
+__device__ __forceinline__ uint32_t colbase(const Column& c)
+{
+    const int sx = ex == 1 ? c.xm : (ex == -1 ? c.xp : c.x);
+    const int sy = ey == 1 ? c.ym : (ey == -1 ? c.yp : c.y);
+    return (uint32_t)((sx * c.Y + sy) * c.Z);
+}
+
+// Wall word of a column: the wall rule of its two wall-adjacent nodes,
+// precomputed from the wall masks (wall_word_kernel).
+//   bits 0-4 : bottom node, directions 5,11,14,15,18: 1 = specular, 0 = bounce-back
+//   bits 5-9 : top node, directions 6,12,13,16,17
+//   bits 10-13: bounce-back of the pushes 12, 13 (bottom) and 11, 14 (top),
+//              which reverse e_x (momentum budget)
+constexpr int kBotDirs[5] = {5, 11, 14, 15, 18};
+constexpr int kTopDirs[5] = {6, 12, 13, 16, 17};
+
+template <int a>
+constexpr int wall_bit()
+{
+    for (int i = 0; i < 5; i++) {
+        if (kBotDirs[i] == a) return i;
+        if (kTopDirs[i] == a) return 5 + i;
+    }
+    return -1;
+}
+
+__device__ __forceinline__ uint32_t load_uniform_u32(const uint32_t* p)
+{
+    return *(const __attribute__((address_space(4))) uint32_t*)p;
 }
 
 // Gather the post-stream populations of node (column c, row z) from the
@@ -114,7 +163,7 @@ __device__ __forceinline__ void pull19(const T* __restrict__ src, long long P, c
     const uint32_t n = c.n0 + (uint32_t)z;
     static_for<0, Q>([&](auto ac) {
         constexpr int a = decltype(ac)::value;
-        uint32_t idx = c.cb[EX[a] + 1][EY[a] + 1] + (uint32_t)(z - EZ[a]);
+        uint32_t idx = colbase<EX[a], EY[a]>(c) + (uint32_t)(z - EZ[a]);
         if constexpr (EZ[a] == 1) idx = (z == 0) ? n : idx;
         if constexpr (EZ[a] == -1) idx = (z == Z - 1) ? n : idx;
         v[a] = src[(long long)a * P + idx];
@@ -129,7 +178,7 @@ __device__ __forceinline__ void pull19(const T* __restrict__ src, long long P, c
                 const int sy = EY[a] == 1 ? c.ym : (EY[a] == -1 ? c.yp : c.y);
                 const int mS = mbot[sx * Y + sy];
                 if (mW == -2 || (mS == -2 && mW == -1))
-                    v[a] = src[(long long)MIR[a] * P + c.cb[EX[a] + 1][EY[a] + 1] + (uint32_t)z];
+                    v[a] = src[(long long)MIR[a] * P + colbase<EX[a], EY[a]>(c) + (uint32_t)z];
                 else
                     v[a] = src[(long long)OPP[a] * P + n];
             }
@@ -144,7 +193,7 @@ __device__ __forceinline__ void pull19(const T* __restrict__ src, long long P, c
                 const int sy = EY[a] == 1 ? c.ym : (EY[a] == -1 ? c.yp : c.y);
                 const int mS = mtop[sx * Y + sy];
                 if (mW == -2 || (mS == -2 && mW == -1))
-                    v[a] = src[(long long)MIR[a] * P + c.cb[EX[a] + 1][EY[a] + 1] + (uint32_t)z];
+                    v[a] = src[(long long)MIR[a] * P + colbase<EX[a], EY[a]>(c) + (uint32_t)z];
                 else
                     v[a] = src[(long long)OPP[a] * P + n];
             }
@@ -159,26 +208,6 @@ __device__ __forceinline__ void load_local19(const T* __restrict__ src, long lon
         constexpr int a = decltype(ac)::value;
         v[a] = src[(long long)a * P + n];
     });
-}
-
-// Sign of each population's x-momentum after the next streaming: -1 when
-// the push of (node, a) is a halfway bounce-back (xcomputations.c:370), so
-// the momentum budget of the next state is sum e_x g * sign.  Only
-// directions with e_x != 0 and e_z pointing into a wall matter.
-__device__ __forceinline__ void bb_flags(const Column& c, int z, int Z, int Y, const int8_t* __restrict__ mbot,
-                                         const int8_t* __restrict__ mtop, bool& f12, bool& f13, bool& f11, bool& f14)
-{
-    f12 = f13 = f11 = f14 = false;
-    if (z == 0) {
-        const bool w = mbot[c.x * Y + c.y] == -1;
-        f12 = w && mbot[c.xp * Y + c.y] == -1;  // a=12 (1,0,-1) pushes to x+1
-        f13 = w && mbot[c.xm * Y + c.y] == -1;  // a=13 (-1,0,-1) pushes to x-1
-    }
-    if (z == Z - 1) {
-        const bool w = mtop[c.x * Y + c.y] == -1;
-        f11 = w && mtop[c.xp * Y + c.y] == -1;  // a=11 (1,0,1)
-        f14 = w && mtop[c.xm * Y + c.y] == -1;  // a=14 (-1,0,1)
-    }
 }
 
 // Deterministic block reduction of the per-lane accumulators: per-plane
@@ -217,11 +246,65 @@ __device__ __forceinline__ void block_reduce_write(const StepArgs& A, double (&a
     }
 }
 
+// Hot-path gather for one wave: lane z of the wave's column segment.
+// All 19 source addresses are a wave-uniform base (SGPRs: direction plane
+// + upstream column - e_z) plus the lane's z * sizeof(T), so each load is a
+// single global_load with a scalar base and one shared VGPR offset.  The
+// wall-adjacent lanes read a harmless in-allocation neighbour first and are
+// then overwritten by the wall rule (uniform decisions from the column's
+// wall word, one active lane).
+template <typename T, bool FIRST>
+__device__ __forceinline__ void gather_wave(const T* __restrict__ src, long long P, const Column& c, uint32_t zoff,
+                                            int zc, int lane, int zc_top, int lane_top, uint32_t ww, T (&v)[Q])
+{
+    const char* s8 = (const char*)src;
+    static_for<0, Q>([&](auto ac) {
+        constexpr int a = decltype(ac)::value;
+        long long e;
+        if constexpr (FIRST)
+            e = (long long)a * P + c.n0;
+        else
+            e = (long long)a * P + (long long)colbase<EX[a], EY[a]>(c) - EZ[a];
+        const char* base = s8 + e * (long long)sizeof(T);
+        v[a] = *(const T*)(base + zoff);
+    });
+    if constexpr (!FIRST) {
+        if (zc == 0) {
+            static_for<0, Q>([&](auto ac) {
+                constexpr int a = decltype(ac)::value;
+                if constexpr (EZ[a] == 1) {
+                    const bool spec = (ww >> wall_bit<a>()) & 1u;
+                    const long long e = spec ? (long long)MIR[a] * P + colbase<EX[a], EY[a]>(c)
+                                             : (long long)OPP[a] * P + c.n0;
+                    if (lane == 0) v[a] = src[e];
+                }
+            });
+        }
+        if (zc == zc_top) {
+            const long long zt = (long long)(zc_top * kWave + lane_top);
+            static_for<0, Q>([&](auto ac) {
+                constexpr int a = decltype(ac)::value;
+                if constexpr (EZ[a] == -1) {
+                    const bool spec = (ww >> wall_bit<a>()) & 1u;
+                    const long long e = spec ? (long long)MIR[a] * P + colbase<EX[a], EY[a]>(c) + zt
+                                             : (long long)OPP[a] * P + c.n0 + zt;
+                    if (lane == lane_top) v[a] = src[e];
+                }
+            });
+        }
+    }
+}
+
 // The fused step: pull-stream + wall rule + BGK collision with force +
 // per-plane statistics + momentum budget of the next state.
 //   FIRST: the source holds post-stream populations (after a load or an
 //          init), so the gather is local instead of a pull.
 //   STATS: accumulate the per-plane sums for the per-step logs.
+// Budget of the next state (the force of the next step, encalc.c:121):
+// mass and x-momentum are conserved by the collision except for the force
+// term, whose x-moment is exactly G, and streaming is a permutation except
+// that a halfway bounce-back reverses e_x:
+//     sum f_next = sum f,  sum e_x f_next = sum (j_x + G) - 2 sum_{BB} e_x g
 template <typename T, bool FIRST, bool STATS>
 __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs A)
 {
@@ -234,8 +317,10 @@ __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs A)
     const long long P = A.g.P;
     const int z = zc * kWave + lane;
     const bool zvalid = z < Z;
+    const int zc_top = (Z - 1) / kWave, lane_top = (Z - 1) % kWave;
+    const uint32_t zoff = (uint32_t)z * (uint32_t)sizeof(T);
     const T* __restrict__ src = (const T*)A.src;
-    T* __restrict__ dst = (T*)A.dst;
+    char* __restrict__ d8 = (char*)A.dst;
 
     double acc[kNAcc];
 #pragma unroll
@@ -251,76 +336,103 @@ __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs A)
 
     for (int grp = blockIdx.x; grp < A.g.ngroups; grp += gridDim.x) {
         const int col = __builtin_amdgcn_readfirstlane(grp * A.g.cpb + slot);
-        if (col >= A.g.ncols || !zvalid) continue;
+        if (col >= A.g.ncols) continue;
         const Column c = make_column(col, X, Y, Z);
-        const uint32_t n = c.n0 + (uint32_t)z;
+        if (!zvalid) continue;
+        const uint32_t ww = FIRST ? 0u : load_uniform_u32(A.wword + col);
         T v[Q];
-        if constexpr (FIRST)
-            load_local19(src, P, n, v);
-        else
-            pull19(src, P, c, z, Z, Y, A.mbot, A.mtop, v);
+        gather_wave<T, FIRST>(src, P, c, zoff, zc, lane, zc_top, lane_top, ww, v);
 
         double mass_node, mom_node, jx_raw;
+        double g11 = 0., g12 = 0., g13 = 0., g14 = 0.;
         if constexpr (std::is_same<T, double>::value) {
             double ux, uy, uz, dens;
             moments_fp64(v, ux, uy, uz, dens);
             const double u0 = ux / dens, u1 = uy / dens, u2 = uz / dens;
             collide_fp64(v, u0, u1, u2, dens, A.omega, G);
             jx_raw = ux;
+            mass_node = dens - 1.0;
             if constexpr (STATS) {
                 const double du = u0 - cu, dv = u1 - cv, dw = u2 - cw;
                 acc[0] += du;
                 acc[1] += dv;
                 acc[2] += dw;
-                acc[3] += dens - 1.0;
+                acc[3] += mass_node;
                 acc[4] += (du * du + dv * dv) + dw * dw;
             }
-            double m = 0.0;
-#pragma unroll
-            for (int a = 0; a < Q; a++) m += v[a];
-            mass_node = m - 1.0;
-            mom_node = ((v[1] + v[7]) + (v[10] + v[11]) + v[12]) - ((v[2] + v[8]) + (v[9] + v[13]) + v[14]);
-            bool f12, f13, f11, f14;
-            bb_flags(c, z, Z, Y, A.mbot, A.mtop, f12, f13, f11, f14);
-            if (f12) mom_node -= 2.0 * v[12];
-            if (f13) mom_node += 2.0 * v[13];
-            if (f11) mom_node -= 2.0 * v[11];
-            if (f14) mom_node += 2.0 * v[14];
+            g11 = v[11];
+            g12 = v[12];
+            g13 = v[13];
+            g14 = v[14];
         } else {
             float drho, u0, u1, u2, jx;
             collide_fp32(v, (float)A.omega, (float)G, drho, u0, u1, u2, jx);
             jx_raw = (double)jx;
+            mass_node = (double)drho;
             if constexpr (STATS) {
                 const double du = (double)u0 - cu, dv = (double)u1 - cv, dw = (double)u2 - cw;
                 acc[0] += du;
                 acc[1] += dv;
                 acc[2] += dw;
-                acc[3] += (double)drho;
+                acc[3] += mass_node;
                 acc[4] += (du * du + dv * dv) + dw * dw;
             }
-            double m = 0.0;
-#pragma unroll
-            for (int a = 0; a < Q; a++) m += (double)v[a];
-            mass_node = m;
-            mom_node = (((double)v[1] + (double)v[7]) + ((double)v[10] + (double)v[11]) + (double)v[12]) -
-                       (((double)v[2] + (double)v[8]) + ((double)v[9] + (double)v[13]) + (double)v[14]);
-            bool f12, f13, f11, f14;
-            bb_flags(c, z, Z, Y, A.mbot, A.mtop, f12, f13, f11, f14);
-            if (f12) mom_node -= 2.0 * (WD[12] + (double)v[12]);
-            if (f13) mom_node += 2.0 * (WD[13] + (double)v[13]);
-            if (f11) mom_node -= 2.0 * (WD[11] + (double)v[11]);
-            if (f14) mom_node += 2.0 * (WD[14] + (double)v[14]);
+            g11 = WD[11] + (double)v[11];
+            g12 = WD[12] + (double)v[12];
+            g13 = WD[13] + (double)v[13];
+            g14 = WD[14] + (double)v[14];
         }
+        mom_node = jx_raw + G;
+        // bounce-back reverses e_x of the populations pushed into a no-slip
+        // wall node from a no-slip wall column (xcomputations.c:370)
+        // the wall word of the column holding the state after this step
+        const uint32_t wn = FIRST ? load_uniform_u32(A.wword + col) : ww;
+        if (zc == 0 && lane == 0)
+            mom_node = mom_node - (((wn >> 10) & 1u) ? 2.0 * g12 : 0.0) + (((wn >> 11) & 1u) ? 2.0 * g13 : 0.0);
+        if (zc == zc_top && lane == lane_top)
+            mom_node = mom_node - (((wn >> 12) & 1u) ? 2.0 * g11 : 0.0) + (((wn >> 13) & 1u) ? 2.0 * g14 : 0.0);
         if constexpr (STATS) acc[5] += jx_raw;
         acc[6] += mass_node;
         acc[7] += mom_node;
 
         static_for<0, Q>([&](auto ac) {
             constexpr int a = decltype(ac)::value;
-            dst[(long long)a * P + n] = v[a];
+            const long long e = (long long)a * P + c.n0;
+            *(T*)(d8 + e * (long long)sizeof(T) + zoff) = v[a];
         });
     }
     block_reduce_write(A, acc, wave, lane, zc, z, STATS);
+}
+
+// Per-column wall words from the wall masks (Appendix-B pull table of the
+// push rule xcomputations.c:368-375, and its bounce-back pushes).
+__global__ __launch_bounds__(256) void wall_word_kernel(const int8_t* __restrict__ mbot,
+                                                        const int8_t* __restrict__ mtop, int X, int Y,
+                                                        uint32_t* __restrict__ ww)
+{
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= X * Y) return;
+    const int x = col / Y, y = col - x * Y;
+    uint32_t w = 0;
+    for (int side = 0; side < 2; side++) {
+        const int8_t* m = side ? mtop : mbot;
+        const int* dirs = side ? kTopDirs : kBotDirs;
+        const int mW = m[x * Y + y];
+        for (int i = 0; i < 5; i++) {
+            const int a = dirs[i];
+            const int sx = (x - EX[a] + X) % X, sy = (y - EY[a] + Y) % Y;
+            const int mS = m[sx * Y + sy];
+            if (mW == -2 || (mS == -2 && mW == -1)) w |= 1u << (5 * side + i);
+        }
+        // pushes that bounce back from this node: e_x = +1 then -1 toward the wall
+        const int ap = side ? 11 : 12, am = side ? 14 : 13;
+        const int xp = (x + 1) % X, xm = (x - 1 + X) % X;
+        if (mW == -1 && m[xp * Y + y] == -1) w |= 1u << (side ? 12 : 10);
+        if (mW == -1 && m[xm * Y + y] == -1) w |= 1u << (side ? 13 : 11);
+        (void)ap;
+        (void)am;
+    }
+    ww[col] = w;
 }
 
 // Moments-only pass over the current state (no collision, no write):
