@@ -61,8 +61,11 @@ struct dnsgpu {
     DevState* st;
     LogRow* ring;
     int ring_cap;
-    int nblk;
-    int nblk_stats;
+    int nblk;       // blocks of the chunk-mapped kernels (moments, statistics)
+    int step_w;     // waves per block of the step kernel
+    int step_nblk;  // blocks of the step kernel
+    int ncu;
+    int step_nblk_max;  // capacity of the partial buffers for the step grid
     int block;
     hipStream_t stream;
     double omega, mu, dt;
@@ -74,17 +77,6 @@ struct dnsgpu {
 
 extern "C" const char* dnsgpu_last_error(void) { return g_err; }
 
-static long long plane_stride(long long n)
-{
-    // Round each direction plane up to 256 elements and stagger
-    // consecutive planes by 4 KiB of extra offset so the 19 load streams
-    // and 19 store streams of a wavefront do not start on the same HBM
-    // channel.  DNSGPU_PLANE_PAD (elements) overrides the stagger.
-    long long pad = 512;
-    const char* e = getenv("DNSGPU_PLANE_PAD");
-    if (e) pad = atoll(e);
-    return ((n + 255) / 256) * 256 + pad;
-}
 
 template <typename K>
 static int occupancy_blocks(K kernel, int block, int* out)
@@ -162,26 +154,34 @@ static int reduce_partials(dnsgpu* h, bool stats, int nblk)
     return 0;
 }
 
+// The step kernel alone, on the handle's stream.
 template <typename T>
-static int launch_step(dnsgpu* h, bool stats, LogRow* row, long long cntr)
+static void enqueue_step_kernel(dnsgpu* h, bool stats)
 {
     const void* src = h->buf[h->cur];
     void* dst = h->buf[1 - h->cur];
-    StepArgs A = step_args(h, src, dst, h->nblk);
-    dim3 grid(h->nblk), block(h->block);
+    StepArgs A = step_args(h, src, dst, h->step_nblk);
+    dim3 grid(h->step_nblk), block(kWave * h->step_w);
+    const size_t lds = stats ? step_lds_bytes(h->g.Z, h->step_w) : 0;
     if (h->poststream) {
         if (stats)
-            hipLaunchKernelGGL((step_kernel<T, true, true>), grid, block, 0, h->stream, A);
+            hipLaunchKernelGGL((step_kernel<T, true, true>), grid, block, lds, h->stream, A);
         else
-            hipLaunchKernelGGL((step_kernel<T, true, false>), grid, block, 0, h->stream, A);
+            hipLaunchKernelGGL((step_kernel<T, true, false>), grid, block, lds, h->stream, A);
     } else {
         if (stats)
-            hipLaunchKernelGGL((step_kernel<T, false, true>), grid, block, 0, h->stream, A);
+            hipLaunchKernelGGL((step_kernel<T, false, true>), grid, block, lds, h->stream, A);
         else
-            hipLaunchKernelGGL((step_kernel<T, false, false>), grid, block, 0, h->stream, A);
+            hipLaunchKernelGGL((step_kernel<T, false, false>), grid, block, lds, h->stream, A);
     }
+}
+
+template <typename T>
+static int launch_step(dnsgpu* h, bool stats, LogRow* row, long long cntr)
+{
+    enqueue_step_kernel<T>(h, stats);
     HIPCHK(hipGetLastError());
-    int rc = reduce_partials(h, stats, h->nblk);
+    int rc = reduce_partials(h, stats, h->step_nblk);
     if (rc) return rc;
     FinalizeArgs F = fin_args(h, stats, row, cntr, 0);
     hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1024), 0, h->stream, F);
@@ -254,8 +254,9 @@ extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
     g.cpb = std::max(1, 4 / g.nzc);
     g.ncols = g.X * g.Y;
     g.ngroups = (g.ncols + g.cpb - 1) / g.cpb;
-    g.P = plane_stride((long long)g.X * g.Y * g.Z);
+    g.total = (long long)g.ncols * Q * g.Z;
     h->block = kWave * g.nzc * g.cpb;
+    h->step_w = step_waves(g.Z);
     h->esize = c->precision == DNSGPU_FP64 ? 8 : 4;
     // omega and the constants exactly as dns-const-q-xycut.c:211-217,
     // wshearstr.c:24, encalc.c:20 evaluate them.
@@ -278,18 +279,31 @@ extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
         return fail(set_err(DNSGPU_EHIP, "stream creation failed"));
     if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
         return fail(set_err(DNSGPU_EHIP, "event creation failed"));
-    // grid: fill the device once (persistent-style), at most one block per group
+    // grids: the chunk-mapped kernels fill the device once; the step kernel
+    // keeps about 4 (FP64) or 8 (FP32) waves per CU in flight, the measured
+    // optimum of the read+write stream (overridden by the autotuner or
+    // DNSGPU_STEP_BLOCKS)
     int occ = 0;
     if (c->precision == DNSGPU_FP64)
-        rc = occupancy_blocks(step_kernel<double, false, true>, h->block, &occ);
+        rc = occupancy_blocks(moments_kernel<double, false>, h->block, &occ);
     else
-        rc = occupancy_blocks(step_kernel<float, false, true>, h->block, &occ);
+        rc = occupancy_blocks(moments_kernel<float, false>, h->block, &occ);
     if (rc) return fail(rc);
-    const char* ob = getenv("DNSGPU_BLOCKS");
-    if (ob) occ = atoi(ob);
     h->nblk = std::max(1, std::min(occ, g.ngroups));
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return fail(set_err(DNSGPU_EHIP, "device query failed"));
+    h->ncu = ncu;
+    const int waves_per_cu = c->precision == DNSGPU_FP64 ? 4 : 8;
+    int sb = (ncu * waves_per_cu + h->step_w - 1) / h->step_w;
+    const char* sbe = getenv("DNSGPU_STEP_BLOCKS");
+    if (sbe) sb = atoi(sbe);
+    h->step_nblk_max = std::max(1, (g.ncols + h->step_w - 1) / h->step_w);
+    h->step_nblk_max = std::min(h->step_nblk_max, 64 * ncu);
+    h->step_nblk = std::max(1, std::min(sb, h->step_nblk_max));
 
-    const size_t popbytes = (size_t)Q * g.P * h->esize;
+    const size_t popbytes = (size_t)g.total * h->esize;
     const size_t xy = (size_t)g.X * g.Y;
     struct {
         void** p;
@@ -302,8 +316,8 @@ extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
         {(void**)&h->wword, xy * 4},
         {(void**)&h->hbot, xy * 2},
         {(void**)&h->htop, xy * 2},
-        {(void**)&h->part, sizeof(double) * kNQ * g.Z * (size_t)h->nblk},
-        {(void**)&h->gpart, sizeof(double) * kNG * (size_t)h->nblk},
+        {(void**)&h->part, sizeof(double) * kNQ * g.Z * (size_t)std::max(h->nblk, h->step_nblk_max)},
+        {(void**)&h->gpart, sizeof(double) * kNG * (size_t)std::max(h->nblk, h->step_nblk_max)},
         {(void**)&h->planes, sizeof(double) * (kNQ * g.Z + kNG)},
         {(void**)&h->shift, sizeof(double) * 3 * g.Z},
         {(void**)&h->means, sizeof(double) * 4 * g.Z},
@@ -649,32 +663,15 @@ extern "C" int dnsgpu_time_steps(dnsgpu* h, int nsteps, int with_stats, double* 
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     HIPCHK(hipEventRecord(h->ev0, h->stream));
     for (int s = 0; s < nsteps; s++) {
-        const void* src = h->buf[h->cur];
-        void* dst = h->buf[1 - h->cur];
-        StepArgs A = step_args(h, src, dst, h->nblk);
         HIPCHK(hipEventRecord(ev[2 * s], h->stream));
-        const bool first = h->poststream != 0;
         const bool st = with_stats != 0;
-#define LAUNCH_STEP(T)                                                                                         \
-    do {                                                                                                       \
-        if (first && st)                                                                                       \
-            hipLaunchKernelGGL((step_kernel<T, true, true>), dim3(h->nblk), dim3(h->block), 0, h->stream, A);  \
-        else if (first)                                                                                        \
-            hipLaunchKernelGGL((step_kernel<T, true, false>), dim3(h->nblk), dim3(h->block), 0, h->stream, A); \
-        else if (st)                                                                                           \
-            hipLaunchKernelGGL((step_kernel<T, false, true>), dim3(h->nblk), dim3(h->block), 0, h->stream, A); \
-        else                                                                                                   \
-            hipLaunchKernelGGL((step_kernel<T, false, false>), dim3(h->nblk), dim3(h->block), 0, h->stream,    \
-                               A);                                                                             \
-    } while (0)
         if (h->c.precision == DNSGPU_FP64)
-            LAUNCH_STEP(double);
+            enqueue_step_kernel<double>(h, st);
         else
-            LAUNCH_STEP(float);
-#undef LAUNCH_STEP
+            enqueue_step_kernel<float>(h, st);
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(ev[2 * s + 1], h->stream));
-        int rc = reduce_partials(h, st, h->nblk);
+        int rc = reduce_partials(h, st, h->step_nblk);
         if (rc) return rc;
         FinalizeArgs F = fin_args(h, st, nullptr, s, 0);
         hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1024), 0, h->stream, F);

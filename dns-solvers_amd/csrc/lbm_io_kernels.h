@@ -21,14 +21,14 @@ __device__ __forceinline__ float from_f<float>(double f, int a) { return (float)
 
 // Populations of the current post-stream state of node (col, z) as f (FP64).
 template <typename T>
-__device__ __forceinline__ void state_f(const T* src, long long P, const Column& c, int z, int Z, int Y,
-                                        const int8_t* mbot, const int8_t* mtop, bool local, double (&f)[Q])
+__device__ __forceinline__ void state_f(const T* src, const Column& c, int z, const int8_t* mbot,
+                                        const int8_t* mtop, bool local, double (&f)[Q])
 {
     T v[Q];
     if (local)
-        load_local19(src, P, c.n0 + (uint32_t)z, v);
+        load_local19(src, c, z, v);
     else
-        pull19(src, P, c, z, Z, Y, mbot, mtop, v);
+        pull19(src, c, z, mbot, mtop, v);
 #pragma unroll
     for (int a = 0; a < Q; a++) f[a] = Store<T>::to_f(v[a], a);
 }
@@ -78,13 +78,12 @@ __global__ __launch_bounds__(256) void save_aos_kernel(AosArgs A)
         const int z = k - 1;
         const Column c = make_column_lane(x * Y + y, X, Y, Z);
         double fs[Q];
-        state_f<T>((const T*)A.cur, A.g.P, c, z, Z, Y, A.mbot, A.mtop, A.local != 0, fs);
+        state_f<T>((const T*)A.cur, c, z, A.mbot, A.mtop, A.local != 0, fs);
         const T* ob = (const T*)(A.local ? A.other : A.cur);
-        const uint32_t n = c.n0 + (uint32_t)z;
 #pragma unroll
         for (int a = 0; a < Q; a++) {
             ft[a] = fs[a];
-            fo[a] = Store<T>::to_f(ob[(long long)a * A.g.P + n], a);
+            fo[a] = Store<T>::to_f(ob[c.cb + (long long)a * Z + z], a);
         }
     }
 }
@@ -104,11 +103,11 @@ __global__ __launch_bounds__(256) void load_aos_kernel(AosArgs A)
         const int y = (int)(cl % Y);
         const int x = A.x0 + (int)(cl / Y);
         const long long src = (cl * (Z + 2) + (z + 1)) * Q;
-        const long long n = ((long long)x * Y + y) * Z + z;
+        const long long cb = ((long long)x * Y + y) * Q * Z + z;
 #pragma unroll
         for (int a = 0; a < Q; a++) {
-            cw[(long long)a * A.g.P + n] = from_f<T>(A.ftemp[src + a], a);
-            ow[(long long)a * A.g.P + n] = from_f<T>(A.f[src + a], a);
+            cw[cb + (long long)a * Z] = from_f<T>(A.ftemp[src + a], a);
+            ow[cb + (long long)a * Z] = from_f<T>(A.f[src + a], a);
         }
     }
 }
@@ -143,7 +142,7 @@ __global__ __launch_bounds__(256) void fields_kernel(FieldArgs A)
         if (k >= 1 && k <= Z) {
             const Column c = make_column_lane(x * Y + y, X, Y, Z);
             double f[Q];
-            state_f<T>((const T*)A.cur, A.g.P, c, k - 1, Z, Y, A.mbot, A.mtop, A.local != 0, f);
+            state_f<T>((const T*)A.cur, c, k - 1, A.mbot, A.mtop, A.local != 0, f);
             moments_fp64(f, ux, uy, uz, dens);
         }
         if (dens > 0.) {
@@ -204,7 +203,7 @@ __global__ __launch_bounds__(kWave* kMaxWaves) void stats_kernel(StatArgs A)
         if (col >= A.g.ncols || !zvalid) continue;
         const Column c = make_column(col, X, Y, Z);
         double f[Q];
-        state_f<T>((const T*)A.cur, A.g.P, c, z, Z, Y, A.mbot, A.mtop, A.local != 0, f);
+        state_f<T>((const T*)A.cur, c, z, A.mbot, A.mtop, A.local != 0, f);
         double u, v, w, dens;
         moments_fp64(f, u, v, w, dens);
         if (A.pass == 1) {
@@ -354,11 +353,12 @@ __global__ __launch_bounds__(256) void init_kernel(InitArgs A)
         } else {
             feq_fp64(u0, u1, u2, f);
         }
+        const long long cb = ((long long)x * Y + y) * Q * Z + z;
 #pragma unroll
         for (int a = 0; a < Q; a++) {
             const T v = from_f<T>(f[a], a);
-            cw[(long long)a * A.g.P + n] = v;
-            ow[(long long)a * A.g.P + n] = v;
+            cw[cb + (long long)a * Z] = v;
+            ow[cb + (long long)a * Z] = v;
         }
     }
 }

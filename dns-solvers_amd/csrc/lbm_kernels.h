@@ -1,24 +1,28 @@
 // lbm_kernels.h -- the fused D3Q19 collide-stream step and its helpers
 // for gfx950 (MI355X).
 //
-// Device layout (structure of arrays, one plane per direction):
-//     pop[a * P + (x*Y + y)*Z + z],  z = 0..Z-1 the fluid rows
-// z (wall-normal) is the fastest index, as k is in the reference's
-// Fb(...) macro (definitions.h:93); the wall rows are not stored.  P is the
-// plane stride in elements (Nx*Ny*Nz rounded up and staggered).
+// Device layout ("column blocks"): for each (x, y) column the 19 direction
+// segments of its Z fluid rows are contiguous,
+//     pop[((x*Y + y)*19 + a)*Z + z],   z = 0..Z-1 the fluid rows,
+// z (wall-normal) fastest as k is in the reference's Fb(...) macro
+// (definitions.h:93); the wall rows are not stored.  A wave that updates a
+// column therefore writes one contiguous block of 19*Z values and reads
+// from the 9 column blocks of its 3x3 (x, y) neighbourhood, which keeps the
+// number of DRAM pages open per wave small (measured on MI355X: fetched
+// bytes within 3% of the algorithmic 19*sizeof(T) per node, against 6-27%
+// excess for one plane per direction).
 //
-// Work decomposition: one wavefront owns one (x, y) column segment of 64
-// wall-normal rows, so every address offset that depends on x and y (the
-// nine neighbour columns of the pull, the wall-mask entries) is
-// wave-uniform and lives in SGPRs; the per-lane part of every load is z.
-// A block holds cpb columns x nzc segments; blocks loop over column groups
-// (persistent-style grid sized to fill the device once).
+// Work decomposition of the step: one wavefront owns one column at a time
+// and walks it in 64-row chunks, lane = row; every address term that
+// depends on x and y (the nine upstream column blocks, the wall word) is
+// wave-uniform and lives in SGPRs.  Blocks of W waves loop over columns
+// (persistent-style grid sized by the autotuner).
 //
 // Streaming is "pull" (A-B): the kernel gathers the post-collision values
 // of the previous step from the 19 upstream neighbours (periodic wrap in x
 // and y, the one-rank halo of xcomputations.c:222-289,435-555), applies the
 // wall rule of xcomputations.c:368-375 in pull form, collides and writes
-// the node's 19 post-collision values contiguously.  The pull rule for the
+// the node's 19 post-collision values.  The pull rule for the
 // wall-adjacent row, node X, direction d with upstream S = X - e_d in the
 // wall row, W = (X.x, X.y, wall), Y = (X.x - e_dx, X.y - e_dy, X.z):
 //     s(W) == -2, or s(S) == -2 and s(W) == -1  ->  g_mirror(d)(Y)  (specular)
@@ -35,26 +39,26 @@ namespace lbm {
 using namespace d3q19;
 
 constexpr int kWave = 64;
-constexpr int kMaxWaves = 8;     // waves per block
-constexpr int kNQ = 5;           // per-plane accumulators: su sv sw sdrho sq
-constexpr int kNG = 3;           // global accumulators: jx, mass-1, budget momentum
+constexpr int kMaxWaves = 8;  // waves per block
+constexpr int kNQ = 5;        // per-plane accumulators: su sv sw sdrho sq
+constexpr int kNG = 3;        // global accumulators: jx, mass-1, budget momentum
 constexpr int kNAcc = kNQ + kNG;
 
 struct Geom {
     int X, Y, Z;
-    int nzc;      // 64-row segments per column
-    int cpb;      // columns per block iteration
-    int ncols;    // X*Y
-    int ngroups;  // ceil(ncols / cpb)
-    long long P;  // plane stride in elements
+    int nzc;          // 64-row chunks per column
+    int cpb;          // columns per block iteration (chunk-mapped kernels)
+    int ncols;        // X*Y
+    int ngroups;      // ceil(ncols / cpb)
+    long long total;  // elements of one population array: ncols*19*Z
 };
 
 struct StepArgs {
     Geom g;
     const void* src;
     void* dst;
-    const int8_t* mbot;  // [X*Y] bottom wall codes (-1 / -2)
-    const int8_t* mtop;  // [X*Y] top wall codes
+    const int8_t* mbot;     // [X*Y] bottom wall codes (-1 / -2)
+    const int8_t* mtop;     // [X*Y] top wall codes
     const uint32_t* wword;  // [X*Y] per-column wall words
     double omega;
     const double* G;      // force used by this step (device scalar)
@@ -77,16 +81,11 @@ struct Store<float> {
     static __device__ __forceinline__ double to_f(float v, int a) { return WD[a] + (double)v; }
 };
 
-__device__ __forceinline__ int wrap_dec(int v, int n) { return v == 0 ? n - 1 : v - 1; }
-__device__ __forceinline__ int wrap_inc(int v, int n) { return v == n - 1 ? 0 : v + 1; }
-
-// Column geometry of one work item.  All fields are wave-uniform; they are
-// forced into SGPRs with readfirstlane so every address below is a scalar
-// base plus the lane's z offset.
+// Column geometry of one work item.
 struct Column {
     int x, y, xm, xp, ym, yp;
     int Y, Z;
-    uint32_t n0;  // own column base (x*Y + y)*Z
+    long long cb;  // own column block: col*19*Z
 };
 
 // Per-thread column (the thread's own column; no uniformity assumed).
@@ -101,11 +100,12 @@ __device__ __forceinline__ Column make_column_lane(int col, int X, int Y, int Z)
     c.yp = c.y == Y - 1 ? 0 : c.y + 1;
     c.Y = Y;
     c.Z = Z;
-    c.n0 = (uint32_t)((c.x * Y + c.y) * Z);
+    c.cb = (long long)col * Q * Z;
     return c;
 }
 
-// Wave-uniform column (col must be the same in every lane of the wave).
+// Wave-uniform column (col must be the same in every lane of the wave);
+// readfirstlane keeps every derived address term in SGPRs.
 __device__ __forceinline__ Column make_column(int col, int X, int Y, int Z)
 {
     Column c;
@@ -117,17 +117,17 @@ __device__ __forceinline__ Column make_column(int col, int X, int Y, int Z)
     c.yp = c.y == Y - 1 ? 0 : c.y + 1;
     c.Y = Y;
     c.Z = Z;
-    c.n0 = (uint32_t)((c.x * Y + c.y) * Z);
+    c.cb = (long long)col * Q * Z;
     return c;
 }
 
-// Base of the column (x - ex, y - ey) that direction (ex, ey) pulls from.
+// Block of the column (x - ex, y - ey) that direction (ex, ey) pulls from.
 template <int ex, int ey>
-__device__ __forceinline__ uint32_t colbase(const Column& c)
+__device__ __forceinline__ long long colbase(const Column& c)
 {
     const int sx = ex == 1 ? c.xm : (ex == -1 ? c.xp : c.x);
     const int sy = ey == 1 ? c.ym : (ey == -1 ? c.yp : c.y);
-    return (uint32_t)((sx * c.Y + sy) * c.Z);
+    return (long long)(sx * c.Y + sy) * Q * c.Z;
 }
 
 // Wall word of a column: the wall rule of its two wall-adjacent nodes,
@@ -155,18 +155,18 @@ __device__ __forceinline__ uint32_t load_uniform_u32(const uint32_t* p)
 }
 
 // Gather the post-stream populations of node (column c, row z) from the
-// post-collision array src (pull), wall rule included.
+// post-collision array src (pull), wall rule included.  Generic per-thread
+// form used by the cadence kernels.
 template <typename T>
-__device__ __forceinline__ void pull19(const T* __restrict__ src, long long P, const Column& c, int z, int Z, int Y,
+__device__ __forceinline__ void pull19(const T* __restrict__ src, const Column& c, int z,
                                        const int8_t* __restrict__ mbot, const int8_t* __restrict__ mtop, T (&v)[Q])
 {
-    const uint32_t n = c.n0 + (uint32_t)z;
+    const int Z = c.Z, Y = c.Y;
     static_for<0, Q>([&](auto ac) {
         constexpr int a = decltype(ac)::value;
-        uint32_t idx = colbase<EX[a], EY[a]>(c) + (uint32_t)(z - EZ[a]);
-        if constexpr (EZ[a] == 1) idx = (z == 0) ? n : idx;
-        if constexpr (EZ[a] == -1) idx = (z == Z - 1) ? n : idx;
-        v[a] = src[(long long)a * P + idx];
+        int zs = z - EZ[a];
+        zs = zs < 0 ? 0 : (zs >= Z ? Z - 1 : zs);
+        v[a] = src[colbase<EX[a], EY[a]>(c) + (long long)a * Z + zs];
     });
     // Wall-adjacent rows: the upstream node is in the wall row.
     if (z == 0) {
@@ -178,9 +178,9 @@ __device__ __forceinline__ void pull19(const T* __restrict__ src, long long P, c
                 const int sy = EY[a] == 1 ? c.ym : (EY[a] == -1 ? c.yp : c.y);
                 const int mS = mbot[sx * Y + sy];
                 if (mW == -2 || (mS == -2 && mW == -1))
-                    v[a] = src[(long long)MIR[a] * P + colbase<EX[a], EY[a]>(c) + (uint32_t)z];
+                    v[a] = src[colbase<EX[a], EY[a]>(c) + (long long)MIR[a] * Z + z];
                 else
-                    v[a] = src[(long long)OPP[a] * P + n];
+                    v[a] = src[c.cb + (long long)OPP[a] * Z + z];
             }
         });
     }
@@ -193,113 +193,95 @@ __device__ __forceinline__ void pull19(const T* __restrict__ src, long long P, c
                 const int sy = EY[a] == 1 ? c.ym : (EY[a] == -1 ? c.yp : c.y);
                 const int mS = mtop[sx * Y + sy];
                 if (mW == -2 || (mS == -2 && mW == -1))
-                    v[a] = src[(long long)MIR[a] * P + colbase<EX[a], EY[a]>(c) + (uint32_t)z];
+                    v[a] = src[colbase<EX[a], EY[a]>(c) + (long long)MIR[a] * Z + z];
                 else
-                    v[a] = src[(long long)OPP[a] * P + n];
+                    v[a] = src[c.cb + (long long)OPP[a] * Z + z];
             }
         });
     }
 }
 
 template <typename T>
-__device__ __forceinline__ void load_local19(const T* __restrict__ src, long long P, uint32_t n, T (&v)[Q])
+__device__ __forceinline__ void load_local19(const T* __restrict__ src, const Column& c, int z, T (&v)[Q])
 {
     static_for<0, Q>([&](auto ac) {
         constexpr int a = decltype(ac)::value;
-        v[a] = src[(long long)a * P + n];
+        v[a] = src[c.cb + (long long)a * c.Z + z];
     });
 }
 
-// Deterministic block reduction of the per-lane accumulators: per-plane
-// sums over the cpb column slots (fixed order), global sums over the whole
-// block (fixed order), written as partials for the fixed-order finishers.
-__device__ __forceinline__ void block_reduce_write(const StepArgs& A, double (&acc)[kNAcc], int wave, int lane,
-                                                   int zc, int z, bool stats)
-{
-    __shared__ double red[kMaxWaves][kNAcc][kWave];
-    const int nw = A.g.nzc * A.g.cpb;
-#pragma unroll
-    for (int q = 0; q < kNAcc; q++) red[wave][q][lane] = acc[q];
-    __syncthreads();
-    if (stats && wave < A.g.nzc) {
-        // wave == zc of slot 0 handles segment zc
-        if (z < A.g.Z) {
-#pragma unroll
-            for (int q = 0; q < kNQ; q++) {
-                double s = 0.0;
-                for (int sl = 0; sl < A.g.cpb; sl++) s += red[sl * A.g.nzc + zc][q][lane];
-                A.part[((long long)q * A.g.Z + z) * A.nblk + blockIdx.x] = s;
-            }
-        }
-    }
-    if (wave == 0) {
-#pragma unroll
-        for (int q = kNQ; q < kNAcc; q++) {
-            if (!stats && q == kNQ) continue;
-            double s = 0.0;
-            for (int w = 0; w < nw; w++) s += red[w][q][lane];
-            // fixed butterfly over the 64 lanes
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
-            if (lane == 0) A.gpart[(long long)(q - kNQ) * A.nblk + blockIdx.x] = s;
-        }
-    }
-}
-
-// Hot-path gather for one wave: lane z of the wave's column segment.
-// All 19 source addresses are a wave-uniform base (SGPRs: direction plane
-// + upstream column - e_z) plus the lane's z * sizeof(T), so each load is a
-// single global_load with a scalar base and one shared VGPR offset.  The
-// wall-adjacent lanes read a harmless in-allocation neighbour first and are
-// then overwritten by the wall rule (uniform decisions from the column's
-// wall word, one active lane).
+// Hot-path gather for one wave: lane z of the chunk.  Every source address
+// is a wave-uniform base (SGPRs: upstream column block + direction segment
+// - e_z) plus the lane's row offset.  The wall-adjacent lanes, whose
+// upstream row is in the wall, read a clamped in-segment row (so no extra
+// cache line is fetched) and are then overwritten by the wall rule: uniform
+// decisions from the column's wall word, one active lane.
 template <typename T, bool FIRST>
-__device__ __forceinline__ void gather_wave(const T* __restrict__ src, long long P, const Column& c, uint32_t zoff,
-                                            int zc, int lane, int zc_top, int lane_top, uint32_t ww, T (&v)[Q])
+__device__ __forceinline__ void gather_wave(const T* __restrict__ src, const Column& c, uint32_t zoff,
+                                            uint32_t zoff_up, uint32_t zoff_dn, bool bot_lane, bool top_lane,
+                                            int z, uint32_t ww, T (&v)[Q])
 {
     const char* s8 = (const char*)src;
+    const long long Z = c.Z;
     static_for<0, Q>([&](auto ac) {
         constexpr int a = decltype(ac)::value;
         long long e;
-        if constexpr (FIRST)
-            e = (long long)a * P + c.n0;
-        else
-            e = (long long)a * P + (long long)colbase<EX[a], EY[a]>(c) - EZ[a];
-        const char* base = s8 + e * (long long)sizeof(T);
-        v[a] = *(const T*)(base + zoff);
+        uint32_t off;
+        if constexpr (FIRST) {
+            e = c.cb + (long long)a * Z;
+            off = zoff;
+        } else {
+            e = colbase<EX[a], EY[a]>(c) + (long long)a * Z - EZ[a];
+            off = EZ[a] == 1 ? zoff_up : (EZ[a] == -1 ? zoff_dn : zoff);
+        }
+        v[a] = *(const T*)(s8 + e * (long long)sizeof(T) + off);
     });
     if constexpr (!FIRST) {
-        if (zc == 0) {
+        if (bot_lane) {
             static_for<0, Q>([&](auto ac) {
                 constexpr int a = decltype(ac)::value;
                 if constexpr (EZ[a] == 1) {
                     const bool spec = (ww >> wall_bit<a>()) & 1u;
-                    const long long e = spec ? (long long)MIR[a] * P + colbase<EX[a], EY[a]>(c)
-                                             : (long long)OPP[a] * P + c.n0;
-                    if (lane == 0) v[a] = src[e];
+                    const long long e = spec ? colbase<EX[a], EY[a]>(c) + (long long)MIR[a] * Z
+                                             : c.cb + (long long)OPP[a] * Z;
+                    v[a] = src[e + z];
                 }
             });
         }
-        if (zc == zc_top) {
-            const long long zt = (long long)(zc_top * kWave + lane_top);
+        if (top_lane) {
             static_for<0, Q>([&](auto ac) {
                 constexpr int a = decltype(ac)::value;
                 if constexpr (EZ[a] == -1) {
                     const bool spec = (ww >> wall_bit<a>()) & 1u;
-                    const long long e = spec ? (long long)MIR[a] * P + colbase<EX[a], EY[a]>(c) + zt
-                                             : (long long)OPP[a] * P + c.n0 + zt;
-                    if (lane == lane_top) v[a] = src[e];
+                    const long long e = spec ? colbase<EX[a], EY[a]>(c) + (long long)MIR[a] * Z
+                                             : c.cb + (long long)OPP[a] * Z;
+                    v[a] = src[e + z];
                 }
             });
         }
     }
 }
+
+// Number of waves per block of the step kernel for a given Z, so the
+// per-wave plane accumulators (kNQ*Z doubles each), the block's shift copy
+// (3*Z doubles) and the static reduction array (kMaxWaves*kNG*kWave
+// doubles) fit in 64 KiB of LDS.
+__host__ __device__ inline int step_waves(int Z)
+{
+    int w = (8192 - kMaxWaves * kNG * kWave - 3 * Z) / (kNQ * Z);
+    if (w > 4) w = 4;
+    if (w < 1) w = 1;
+    return w;
+}
+
+__host__ __device__ inline size_t step_lds_bytes(int Z, int W) { return sizeof(double) * ((size_t)W * kNQ * Z + 3 * Z); }
 
 // The fused step: pull-stream + wall rule + BGK collision with force +
 // per-plane statistics + momentum budget of the next state.
 //   FIRST: the source holds post-stream populations (after a load or an
 //          init), so the gather is local instead of a pull.
-//   STATS: accumulate the per-plane sums for the per-step logs.
+//   STATS: accumulate the per-plane sums for the per-step logs (in each
+//          wave's private LDS slice, so the sums are deterministic).
 // Budget of the next state (the force of the next step, encalc.c:121):
 // mass and x-momentum are conserved by the collision except for the force
 // term, whose x-moment is exactly G, and streaming is a permutation except
@@ -308,104 +290,128 @@ __device__ __forceinline__ void gather_wave(const T* __restrict__ src, long long
 template <typename T, bool FIRST, bool STATS>
 __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs A)
 {
+    extern __shared__ double lds[];  // STATS: [W][kNQ][Z] plane sums, then [3][Z] shifts
+    __shared__ double red[kMaxWaves][kNG][kWave];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int nzc = A.g.nzc;
-    const int zc = wave % nzc;
-    const int slot = wave / nzc;
-    const int X = A.g.X, Y = A.g.Y, Z = A.g.Z;
-    const long long P = A.g.P;
-    const int z = zc * kWave + lane;
-    const bool zvalid = z < Z;
-    const int zc_top = (Z - 1) / kWave, lane_top = (Z - 1) % kWave;
-    const uint32_t zoff = (uint32_t)z * (uint32_t)sizeof(T);
+    const int W = blockDim.x / kWave;
+    const int X = A.g.X, Y = A.g.Y, Z = A.g.Z, nzc = A.g.nzc;
+    const int ctop = (Z - 1) / kWave, ltop = (Z - 1) % kWave;
     const T* __restrict__ src = (const T*)A.src;
     char* __restrict__ d8 = (char*)A.dst;
+    double* mine = lds + (size_t)wave * kNQ * Z;
+    double* shl = lds + (size_t)W * kNQ * Z;
 
-    double acc[kNAcc];
-#pragma unroll
-    for (int q = 0; q < kNAcc; q++) acc[q] = 0.0;
-
+    if constexpr (STATS) {
+        for (int i = lane; i < kNQ * Z; i += kWave) mine[i] = 0.0;
+        for (int i = threadIdx.x; i < 3 * Z; i += blockDim.x) shl[i] = A.shift[i];
+        __syncthreads();
+    }
+    double gjx = 0.0, gmass = 0.0, gmom = 0.0;
     const double G = *A.G;
-    double cu = 0.0, cv = 0.0, cw = 0.0;
-    if (STATS && zvalid) {
-        cu = A.shift[z];
-        cv = A.shift[Z + z];
-        cw = A.shift[2 * Z + z];
-    }
 
-    for (int grp = blockIdx.x; grp < A.g.ngroups; grp += gridDim.x) {
-        const int col = __builtin_amdgcn_readfirstlane(grp * A.g.cpb + slot);
-        if (col >= A.g.ncols) continue;
+    for (int it = blockIdx.x * W + wave; it < A.g.ncols; it += gridDim.x * W) {
+        const int col = __builtin_amdgcn_readfirstlane(it);
         const Column c = make_column(col, X, Y, Z);
-        if (!zvalid) continue;
-        const uint32_t ww = FIRST ? 0u : load_uniform_u32(A.wword + col);
-        T v[Q];
-        gather_wave<T, FIRST>(src, P, c, zoff, zc, lane, zc_top, lane_top, ww, v);
+        const uint32_t ww = load_uniform_u32(A.wword + col);
+        for (int ch = 0; ch < nzc; ch++) {
+            const int z = ch * kWave + lane;
+            if (z >= Z) break;
+            const bool bot_lane = ch == 0 && lane == 0;
+            const bool top_lane = ch == ctop && lane == ltop;
+            const uint32_t zoff = (uint32_t)z * (uint32_t)sizeof(T);
+            const uint32_t zoff_up = (uint32_t)(z == 0 ? 1 : z) * (uint32_t)sizeof(T);
+            const uint32_t zoff_dn = (uint32_t)(z == Z - 1 ? Z - 2 : z) * (uint32_t)sizeof(T);
+            T v[Q];
+            gather_wave<T, FIRST>(src, c, zoff, zoff_up, zoff_dn, bot_lane, top_lane, z, ww, v);
 
-        double mass_node, mom_node, jx_raw;
-        double g11 = 0., g12 = 0., g13 = 0., g14 = 0.;
-        if constexpr (std::is_same<T, double>::value) {
-            double ux, uy, uz, dens;
-            moments_fp64(v, ux, uy, uz, dens);
-            const double u0 = ux / dens, u1 = uy / dens, u2 = uz / dens;
-            collide_fp64(v, u0, u1, u2, dens, A.omega, G);
-            jx_raw = ux;
-            mass_node = dens - 1.0;
-            if constexpr (STATS) {
-                const double du = u0 - cu, dv = u1 - cv, dw = u2 - cw;
-                acc[0] += du;
-                acc[1] += dv;
-                acc[2] += dw;
-                acc[3] += mass_node;
-                acc[4] += (du * du + dv * dv) + dw * dw;
+            double mass_node, jx_raw, g11, g12, g13, g14;
+            if constexpr (std::is_same<T, double>::value) {
+                double ux, uy, uz, dens;
+                moments_fp64(v, ux, uy, uz, dens);
+                const double u0 = ux / dens, u1 = uy / dens, u2 = uz / dens;
+                collide_fp64(v, u0, u1, u2, dens, A.omega, G);
+                jx_raw = ux;
+                mass_node = dens - 1.0;
+                if constexpr (STATS) {
+                    const double du = u0 - shl[z], dv = u1 - shl[Z + z], dw = u2 - shl[2 * Z + z];
+                    mine[0 * Z + z] += du;
+                    mine[1 * Z + z] += dv;
+                    mine[2 * Z + z] += dw;
+                    mine[3 * Z + z] += mass_node;
+                    mine[4 * Z + z] += (du * du + dv * dv) + dw * dw;
+                }
+                g11 = v[11];
+                g12 = v[12];
+                g13 = v[13];
+                g14 = v[14];
+            } else {
+                float drho, u0, u1, u2, jx;
+                collide_fp32(v, (float)A.omega, (float)G, drho, u0, u1, u2, jx);
+                jx_raw = (double)jx;
+                mass_node = (double)drho;
+                if constexpr (STATS) {
+                    const double du = (double)u0 - shl[z], dv = (double)u1 - shl[Z + z],
+                                 dw = (double)u2 - shl[2 * Z + z];
+                    mine[0 * Z + z] += du;
+                    mine[1 * Z + z] += dv;
+                    mine[2 * Z + z] += dw;
+                    mine[3 * Z + z] += mass_node;
+                    mine[4 * Z + z] += (du * du + dv * dv) + dw * dw;
+                }
+                g11 = WD[11] + (double)v[11];
+                g12 = WD[12] + (double)v[12];
+                g13 = WD[13] + (double)v[13];
+                g14 = WD[14] + (double)v[14];
             }
-            g11 = v[11];
-            g12 = v[12];
-            g13 = v[13];
-            g14 = v[14];
-        } else {
-            float drho, u0, u1, u2, jx;
-            collide_fp32(v, (float)A.omega, (float)G, drho, u0, u1, u2, jx);
-            jx_raw = (double)jx;
-            mass_node = (double)drho;
-            if constexpr (STATS) {
-                const double du = (double)u0 - cu, dv = (double)u1 - cv, dw = (double)u2 - cw;
-                acc[0] += du;
-                acc[1] += dv;
-                acc[2] += dw;
-                acc[3] += mass_node;
-                acc[4] += (du * du + dv * dv) + dw * dw;
-            }
-            g11 = WD[11] + (double)v[11];
-            g12 = WD[12] + (double)v[12];
-            g13 = WD[13] + (double)v[13];
-            g14 = WD[14] + (double)v[14];
+            double mom_node = jx_raw + G;
+            // bounce-back reverses e_x of the populations pushed into a
+            // no-slip wall node from a no-slip wall column (xcomputations.c:370)
+            if (bot_lane)
+                mom_node = mom_node - (((ww >> 10) & 1u) ? 2.0 * g12 : 0.0) + (((ww >> 11) & 1u) ? 2.0 * g13 : 0.0);
+            if (top_lane)
+                mom_node = mom_node - (((ww >> 12) & 1u) ? 2.0 * g11 : 0.0) + (((ww >> 13) & 1u) ? 2.0 * g14 : 0.0);
+            if constexpr (STATS) gjx += jx_raw;
+            gmass += mass_node;
+            gmom += mom_node;
+
+            static_for<0, Q>([&](auto ac) {
+                constexpr int a = decltype(ac)::value;
+                const long long e = c.cb + (long long)a * Z;
+                *(T*)(d8 + e * (long long)sizeof(T) + zoff) = v[a];
+            });
         }
-        mom_node = jx_raw + G;
-        // bounce-back reverses e_x of the populations pushed into a no-slip
-        // wall node from a no-slip wall column (xcomputations.c:370)
-        // the wall word of the column holding the state after this step
-        const uint32_t wn = FIRST ? load_uniform_u32(A.wword + col) : ww;
-        if (zc == 0 && lane == 0)
-            mom_node = mom_node - (((wn >> 10) & 1u) ? 2.0 * g12 : 0.0) + (((wn >> 11) & 1u) ? 2.0 * g13 : 0.0);
-        if (zc == zc_top && lane == lane_top)
-            mom_node = mom_node - (((wn >> 12) & 1u) ? 2.0 * g11 : 0.0) + (((wn >> 13) & 1u) ? 2.0 * g14 : 0.0);
-        if constexpr (STATS) acc[5] += jx_raw;
-        acc[6] += mass_node;
-        acc[7] += mom_node;
-
-        static_for<0, Q>([&](auto ac) {
-            constexpr int a = decltype(ac)::value;
-            const long long e = (long long)a * P + c.n0;
-            *(T*)(d8 + e * (long long)sizeof(T) + zoff) = v[a];
-        });
     }
-    block_reduce_write(A, acc, wave, lane, zc, z, STATS);
+
+    // Deterministic reductions: plane sums over the block's waves in wave
+    // order, global sums over waves then a fixed butterfly over lanes.
+    red[wave][0][lane] = gjx;
+    red[wave][1][lane] = gmass;
+    red[wave][2][lane] = gmom;
+    __syncthreads();
+    if constexpr (STATS) {
+        for (int i = threadIdx.x; i < kNQ * Z; i += blockDim.x) {
+            double s = 0.0;
+            for (int w = 0; w < W; w++) s += lds[(size_t)w * kNQ * Z + i];
+            const int q = i / Z, z = i - q * Z;
+            A.part[((long long)q * Z + z) * A.nblk + blockIdx.x] = s;
+        }
+    }
+    if (wave == 0) {
+#pragma unroll
+        for (int q = 0; q < kNG; q++) {
+            if (!STATS && q == 0) continue;
+            double s = 0.0;
+            for (int w = 0; w < W; w++) s += red[w][q][lane];
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+            if (lane == 0) A.gpart[(long long)q * A.nblk + blockIdx.x] = s;
+        }
+    }
 }
 
-// Per-column wall words from the wall masks (Appendix-B pull table of the
-// push rule xcomputations.c:368-375, and its bounce-back pushes).
+// Per-column wall words from the wall masks (the pull table of the push
+// rule xcomputations.c:368-375, and its bounce-back pushes).
 __global__ __launch_bounds__(256) void wall_word_kernel(const int8_t* __restrict__ mbot,
                                                         const int8_t* __restrict__ mtop, int X, int Y,
                                                         uint32_t* __restrict__ ww)
@@ -425,14 +431,42 @@ __global__ __launch_bounds__(256) void wall_word_kernel(const int8_t* __restrict
             if (mW == -2 || (mS == -2 && mW == -1)) w |= 1u << (5 * side + i);
         }
         // pushes that bounce back from this node: e_x = +1 then -1 toward the wall
-        const int ap = side ? 11 : 12, am = side ? 14 : 13;
         const int xp = (x + 1) % X, xm = (x - 1 + X) % X;
         if (mW == -1 && m[xp * Y + y] == -1) w |= 1u << (side ? 12 : 10);
         if (mW == -1 && m[xm * Y + y] == -1) w |= 1u << (side ? 13 : 11);
-        (void)ap;
-        (void)am;
     }
     ww[col] = w;
+}
+
+// Per-lane accumulators of the chunk-mapped kernels (one wave per
+// (column, 64-row chunk)): per-plane sums over the cpb column slots in
+// fixed order, global sums in a fixed order, written as partials.
+__device__ __forceinline__ void block_reduce_write(const StepArgs& A, double (&acc)[kNAcc], int wave, int lane,
+                                                   int zc, int z)
+{
+    __shared__ double red[kMaxWaves][kNAcc][kWave];
+    const int nw = A.g.nzc * A.g.cpb;
+#pragma unroll
+    for (int q = 0; q < kNAcc; q++) red[wave][q][lane] = acc[q];
+    __syncthreads();
+    if (wave < A.g.nzc && z < A.g.Z) {
+#pragma unroll
+        for (int q = 0; q < kNQ; q++) {
+            double s = 0.0;
+            for (int sl = 0; sl < A.g.cpb; sl++) s += red[sl * A.g.nzc + zc][q][lane];
+            A.part[((long long)q * A.g.Z + z) * A.nblk + blockIdx.x] = s;
+        }
+    }
+    if (wave == 0) {
+#pragma unroll
+        for (int q = kNQ; q < kNAcc; q++) {
+            double s = 0.0;
+            for (int w = 0; w < nw; w++) s += red[w][q][lane];
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+            if (lane == 0) A.gpart[(long long)(q - kNQ) * A.nblk + blockIdx.x] = s;
+        }
+    }
 }
 
 // Moments-only pass over the current state (no collision, no write):
@@ -447,7 +481,6 @@ __global__ __launch_bounds__(kWave* kMaxWaves) void moments_kernel(StepArgs A)
     const int zc = wave % nzc;
     const int slot = wave / nzc;
     const int X = A.g.X, Y = A.g.Y, Z = A.g.Z;
-    const long long P = A.g.P;
     const int z = zc * kWave + lane;
     const bool zvalid = z < Z;
     const T* __restrict__ src = (const T*)A.src;
@@ -464,12 +497,11 @@ __global__ __launch_bounds__(kWave* kMaxWaves) void moments_kernel(StepArgs A)
         const int col = __builtin_amdgcn_readfirstlane(grp * A.g.cpb + slot);
         if (col >= A.g.ncols || !zvalid) continue;
         const Column c = make_column(col, X, Y, Z);
-        const uint32_t n = c.n0 + (uint32_t)z;
         T v[Q];
         if constexpr (LOCAL)
-            load_local19(src, P, n, v);
+            load_local19(src, c, z, v);
         else
-            pull19(src, P, c, z, Z, Y, A.mbot, A.mtop, v);
+            pull19(src, c, z, A.mbot, A.mtop, v);
         double f[Q];
 #pragma unroll
         for (int a = 0; a < Q; a++) f[a] = Store<T>::to_f(v[a], a);
@@ -490,11 +522,10 @@ __global__ __launch_bounds__(kWave* kMaxWaves) void moments_kernel(StepArgs A)
         }
         acc[4] += (du * du + dv * dv) + dw * dw;
         acc[5] += ux;
-        // the state itself is the "next state" for the budget
         acc[6] += 0.0;
         acc[7] += ux;
     }
-    block_reduce_write(A, acc, wave, lane, zc, z, true);
+    block_reduce_write(A, acc, wave, lane, zc, z);
 }
 
 // Fixed-order reduction of the partials: block b sums row b of a
@@ -517,8 +548,8 @@ __global__ __launch_bounds__(256) void reduce_rows_kernel(const double* __restri
 
 // Device-resident per-handle scalars.
 struct DevState {
-    double G;          // force of the next step
-    double rho_mean;   // V.rho of the last logged step
+    double G;         // force of the next step
+    double rho_mean;  // V.rho of the last logged step
     double pad[6];
 };
 

@@ -1,0 +1,348 @@
+/*
+ * dnsrun.c -- host driver of the MI355X channel-flow solver.
+ *
+ * Mirrors the reference driver main() (DNS-solver/src/dns-const-q-xycut.c:
+ * 13-576) on one GPU: the same case inputs (read at run time, dnscase.h),
+ * the same '-s' (rest start, :385-398) and '-p' (restart from lbf.TSTR and
+ * force.TSTR, :399-454) modes, the same loop of up+1 steps (:485-519) with
+ * the seven per-step logs, the cadence outputs every utime steps
+ * (vel/den, turb-fld, uavg-temp) and checkpoints every 10*utime steps
+ * (lbf, force), and at the end 2dresults-*.dat and report.code (:522-541).
+ * All physics runs in libdnsgpu.so behind the C ABI of include/dnsgpu.h;
+ * this file only moves data at cadence and writes files.
+ *
+ * usage: dnsrun -s|-p [case-file]      (case file default: dns.case)
+ */
+#define _POSIX_C_SOURCE 200809L
+#define _FILE_OFFSET_BITS 64
+#include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <sys/stat.h>
+#include <time.h>
+
+#include "../../include/dnsgpu.h"
+#include "dnscase.h"
+#include "dnsio.h"
+
+static double now_s(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+static void die_api(const char *what, int rc)
+{
+    if (rc == DNSGPU_ENAN) {
+        /* wshearstr.c:126-130 */
+        fprintf(stderr, "Eroor: NAN, found in wall shear stress\n");
+        abort();
+    }
+    fprintf(stderr, "dnsrun: %s failed (%d): %s\n", what, rc, dnsgpu_last_error());
+    exit(1);
+}
+
+#define API(call)                          \
+    do {                                   \
+        int rc_ = (call);                  \
+        if (rc_) die_api(#call, rc_);      \
+    } while (0)
+
+/* x-planes per host slab so one slab of both checkpoint halves stays near
+ * 256 MiB. */
+static int slab_planes(const dns_run_case *c)
+{
+    const size_t plane = (size_t)c->ydv * (c->zdv + 2) * 19 * sizeof(double) * 2;
+    size_t n = (256u << 20) / plane;
+    if (n < 1) n = 1;
+    if (n > (size_t)c->xdv) n = (size_t)c->xdv;
+    return (int)n;
+}
+
+static void write_checkpoint(dnsgpu *h, const dns_run_case *c, long tl)
+{
+    char name[64], path[1024];
+    snprintf(name, sizeof name, "lbf.%.4ld", tl);
+    dnsio_path(path, sizeof path, c->outdir, name);
+    FILE *fh = fopen(path, "wb");
+    if (!fh) {
+        fprintf(stderr, "dnsrun: cannot write %s\n", path);
+        exit(1);
+    }
+    const int zd = c->zdv + 2, sp = slab_planes(c);
+    const size_t plane = (size_t)c->ydv * zd * 19;
+    const off_t half = (off_t)c->xdv * plane * sizeof(double);
+    double *a = malloc(sp * plane * sizeof(double)), *b = malloc(sp * plane * sizeof(double));
+    if (!a || !b) {
+        fprintf(stderr, "dnsrun: out of host memory\n");
+        exit(1);
+    }
+    for (int x0 = 0; x0 < c->xdv; x0 += sp) {
+        int nx = c->xdv - x0 < sp ? c->xdv - x0 : sp;
+        API(dnsgpu_save_aos(h, x0, nx, a, b));
+        const size_t nb = nx * plane * sizeof(double);
+        if (fseeko(fh, (off_t)x0 * plane * sizeof(double), SEEK_SET) || fwrite(a, 1, nb, fh) != nb ||
+            fseeko(fh, half + (off_t)x0 * plane * sizeof(double), SEEK_SET) || fwrite(b, 1, nb, fh) != nb) {
+            fprintf(stderr, "dnsrun: write error on %s\n", path);
+            exit(1);
+        }
+    }
+    fclose(fh);
+    free(a);
+    free(b);
+    double G;
+    API(dnsgpu_get_force(h, &G));
+    if (dnsio_write_force(c->outdir, tl, G)) {
+        fprintf(stderr, "dnsrun: cannot write force.%.4ld\n", tl);
+        exit(1);
+    }
+    printf("writing V.force=%f num written=%d myrank=%d\n", G, 1, 0);
+}
+
+static void read_checkpoint(dnsgpu *h, const dns_run_case *c, long tl)
+{
+    char name[64], path[1024];
+    snprintf(name, sizeof name, "lbf.%.4ld", tl);
+    dnsio_path(path, sizeof path, c->outdir, name);
+    FILE *fh = fopen(path, "rb");
+    if (!fh) {
+        fprintf(stderr, "dnsrun: cannot open %s\n", path);
+        exit(1);
+    }
+    const int zd = c->zdv + 2, sp = slab_planes(c);
+    const size_t plane = (size_t)c->ydv * zd * 19;
+    const off_t half = (off_t)c->xdv * plane * sizeof(double);
+    struct stat st;
+    if (fstat(fileno(fh), &st) || st.st_size != 2 * half) {
+        /* the reference reads a short file silently (dns-const-q-xycut.c:
+         * 405-415) and later dies on 0/0; reject it up front */
+        fprintf(stderr, "dnsrun: %s has %lld bytes, expected %lld (2 x %d x %d x %d x 19 doubles)\n", path,
+                (long long)st.st_size, (long long)(2 * half), c->xdv, c->ydv, zd);
+        exit(1);
+    }
+    double *a = malloc(sp * plane * sizeof(double)), *b = malloc(sp * plane * sizeof(double));
+    if (!a || !b) {
+        fprintf(stderr, "dnsrun: out of host memory\n");
+        exit(1);
+    }
+    for (int x0 = 0; x0 < c->xdv; x0 += sp) {
+        int nx = c->xdv - x0 < sp ? c->xdv - x0 : sp;
+        const size_t nb = nx * plane * sizeof(double);
+        if (fseeko(fh, (off_t)x0 * plane * sizeof(double), SEEK_SET) || fread(a, 1, nb, fh) != nb ||
+            fseeko(fh, half + (off_t)x0 * plane * sizeof(double), SEEK_SET) || fread(b, 1, nb, fh) != nb) {
+            fprintf(stderr, "dnsrun: read error on %s\n", path);
+            exit(1);
+        }
+        API(dnsgpu_load_aos(h, x0, nx, a, b));
+    }
+    API(dnsgpu_commit_load(h));
+    fclose(fh);
+    free(a);
+    free(b);
+}
+
+static void write_fields(dnsgpu *h, const dns_run_case *c, int tl)
+{
+    char name[64], pv[1024], pd[1024];
+    snprintf(name, sizeof name, "vel.%.4d", tl);
+    dnsio_path(pv, sizeof pv, c->outdir, name);
+    snprintf(name, sizeof name, "den.%.4d", tl);
+    dnsio_path(pd, sizeof pd, c->outdir, name);
+    FILE *fv = fopen(pv, "wb"), *fd = fopen(pd, "wb");
+    if (!fv || !fd) {
+        fprintf(stderr, "dnsrun: cannot write %s / %s\n", pv, pd);
+        exit(1);
+    }
+    const int zd = c->zdv + 2, sp = slab_planes(c);
+    const size_t nn = (size_t)sp * c->ydv * zd;
+    double *vel = malloc(nn * 3 * sizeof(double)), *den = malloc(nn * sizeof(double));
+    if (!vel || !den) {
+        fprintf(stderr, "dnsrun: out of host memory\n");
+        exit(1);
+    }
+    for (int x0 = 0; x0 < c->xdv; x0 += sp) {
+        int nx = c->xdv - x0 < sp ? c->xdv - x0 : sp;
+        const size_t n = (size_t)nx * c->ydv * zd;
+        API(dnsgpu_fields(h, x0, nx, vel, den));
+        if (fwrite(vel, sizeof(double), 3 * n, fv) != 3 * n || fwrite(den, sizeof(double), n, fd) != n) {
+            fprintf(stderr, "dnsrun: write error on vel/den\n");
+            exit(1);
+        }
+    }
+    fclose(fv);
+    fclose(fd);
+    free(vel);
+    free(den);
+}
+
+static void write_statistics(dnsgpu *h, const dns_run_case *c, long ts)
+{
+    const int zd = c->zdv + 2;
+    double *st = malloc(9 * (size_t)zd * sizeof(double));
+    if (!st) exit(1);
+    API(dnsgpu_statistics(h, st));
+    if (dnsio_write_turbfld(c->outdir, ts, st, zd)) {
+        fprintf(stderr, "dnsrun: cannot write turb-fld\n");
+        exit(1);
+    }
+    free(st);
+}
+
+static void write_twoD(dnsgpu *h, const dns_run_case *c)
+{
+    /* output_twoD.c:5: column (xd/2, yd/2) of the one-rank layout, whose
+     * ghost plane/row (index xdv / ydv) is plane/row 0 */
+    const int zd = c->zdv + 2;
+    const int x = ((c->xdv + 1) / 2) % c->xdv, y = ((c->ydv + 1) / 2) % c->ydv;
+    const size_t n = (size_t)c->ydv * zd;
+    double *vel = malloc(n * 3 * sizeof(double)), *den = malloc(n * sizeof(double));
+    double *jj = calloc((size_t)zd, sizeof(double));
+    if (!vel || !den || !jj) exit(1);
+    API(dnsgpu_fields(h, x, 1, vel, den));
+    for (int k = 1; k < zd - 1; k++) jj[k] = vel[((size_t)y * zd + k) * 3 + 0];
+    if (dnsio_write_twoD(c->outdir, jj, zd, c->ub)) {
+        fprintf(stderr, "dnsrun: cannot write 2dresults\n");
+        exit(1);
+    }
+    free(vel);
+    free(den);
+    free(jj);
+}
+
+static void write_report(const dns_run_case *c, double omega, double seconds, double utime)
+{
+    char path[1024];
+    dnsio_path(path, sizeof path, c->outdir, "report.code");
+    FILE *tm = fopen(path, "w");
+    if (!tm) return;
+    /* dns-const-q-xycut.c:526-539 with the one-rank extents xd = xdv+1,
+     * yd = ydv+1 and G = Fx = 0 as printed there */
+    fprintf(tm, "xd=%d,yd=%d,zd+2=%d,Reb=%f,tau=%f,G=%.10f,Ub=%f,CFLnumber=%f\n", c->xdv + 1, c->ydv + 1,
+            c->zdv + 2, c->Rebs, omega, 0.0, c->ub, c->CFL);
+    fprintf(tm, "number of processors: %d\n", 1);
+    fprintf(tm, "Executed File's Name: %s\n", __FILE__);
+    fprintf(tm, "Compilation Option: xcut\n");
+    if (c->texture == 2) fprintf(tm, "slip option: RSLIP\n");
+    if (c->texture == 1) fprintf(tm, "slip option: PSLIP\n");
+    fprintf(tm, "Total computational time is: %f s\n", seconds);
+    fprintf(tm, "Unit non-dimensional time is: %f iterations\n", utime);
+    fprintf(tm, "Starting time: %d\n", c->TSTR);
+    fprintf(tm, "Current Time is: %d\n", c->TEND);
+    fprintf(tm, "Device: MI355X (gfx950), precision: %s, forcing: %s\n", c->precision ? "fp32-shifted" : "fp64",
+            c->forcing ? "constant force" : "constant flow rate");
+    fclose(tm);
+}
+
+int main(int argc, char *argv[])
+{
+    if (argc < 2 || argv[1][0] != '-' || !argv[1][1]) {
+        fprintf(stderr, "usage: %s -s|-p [case-file]\n", argv[0]);
+        return 2;
+    }
+    const char ch = argv[1][1];
+    const char *casefile = argc > 2 ? argv[2] : "dns.case";
+    dns_run_case c;
+    dns_case_defaults(&c);
+    char err[512];
+    if (dns_case_read(casefile, &c, err, sizeof err)) {
+        fprintf(stderr, "dnsrun: %s\n", err);
+        return 2;
+    }
+    if (ch != 's' && ch != 'p') {
+        /* dns-const-q-xycut.c:455-460 */
+        perror("not a good file name");
+        return -1;
+    }
+    mkdir(c.outdir, 0755);
+
+    /* dns-const-q-xycut.c:211-237 */
+    const int zd = c.zdv + 2;
+    double utime = ((double)(0.5 * (zd - 2.) / c.CFL));
+    utime = (int)utime;
+    long up = (long)(c.TEND - c.TSTR) * (long)utime;
+    if (c.ITER >= 0) up = c.ITER;
+    const int ut = (int)utime;
+
+    dns_case dc;
+    memset(&dc, 0, sizeof dc);
+    dc.xdv = c.xdv;
+    dc.ydv = c.ydv;
+    dc.zdv = c.zdv;
+    dc.Rebs = c.Rebs;
+    dc.ub = c.ub;
+    dc.CFL = c.CFL;
+    dc.TSTR = c.TSTR;
+    dc.forcing = c.forcing;
+    dc.force = c.force;
+    dc.precision = c.precision;
+    dc.omega = c.omega;
+    dnsgpu *h = NULL;
+    API(dnsgpu_create(&dc, &h));
+    const double omega = c.omega > 0 ? c.omega : 1. / ((3. / c.Rebs) * c.ub * (zd - 2.) + 0.5);
+
+    int8_t *mb = malloc((size_t)c.xdv * c.ydv), *mt = malloc((size_t)c.xdv * c.ydv);
+    if (!mb || !mt) return 1;
+    dns_case_masks(&c, mb, mt);
+    API(dnsgpu_set_walls(h, mb, mt));
+    free(mb);
+    free(mt);
+
+    const int TM = c.TSTR;
+    if (ch == 's') {
+        if (c.init == 0) {
+            API(dnsgpu_init_rest(h));
+        } else {
+            dns_init in = {c.init, c.init_uc, c.init_amp, c.init_seed, c.init_modes};
+            API(dnsgpu_init_field(h, &in));
+        }
+    } else {
+        fprintf(stderr, "TRYING TO READ DATA FROM FILES\n");
+        read_checkpoint(h, &c, c.TSTR);
+        double fr = 0.0;
+        if (dnsio_read_force(c.outdir, c.TSTR, &fr) == 0)
+            printf("V.force=%f myrank=%d was read\n", fr, 0); /* informational: encalc recomputes it */
+        fprintf(stderr, "END OF READING DATA FROM FILES\n");
+        fprintf(stderr, "STARTING FROM %d\n", c.TSTR);
+    }
+
+    dns_logs L;
+    if (dnsio_logs_open(&L, c.outdir)) {
+        fprintf(stderr, "dnsrun: cannot open the log files in %s\n", c.outdir);
+        return 1;
+    }
+    dns_steplog *rows = malloc(sizeof(dns_steplog) * (size_t)ut);
+    if (!rows) return 1;
+
+    const double t4 = now_s();
+    int printed_rho = 0;
+    for (long ss = 0; ss <= up;) {
+        if (!(ss % ut)) {
+            const long tl = TM + ss / ut;
+            if (!(ss % (10 * ut))) write_checkpoint(h, &c, tl);
+            write_fields(h, &c, (int)tl);
+            write_statistics(h, &c, tl);
+        }
+        long next = (ss / ut + 1) * (long)ut;
+        if (next > up + 1) next = up + 1;
+        const int n = (int)(next - ss);
+        API(dnsgpu_step(h, ss, n, rows));
+        if (!printed_rho) {
+            printf("Average Density in the channel at start: %f  myrank=%d\n", rows[0].rho, 0);
+            printed_rho = 1;
+        }
+        if (dnsio_logs_append(&L, rows, n)) {
+            fprintf(stderr, "dnsrun: log write failed\n");
+            return 1;
+        }
+        ss = next;
+    }
+    const double t2 = now_s();
+    dnsio_logs_close(&L);
+    write_twoD(h, &c);
+    write_report(&c, omega, t2 - t4, utime);
+    free(rows);
+    dnsgpu_destroy(h);
+    return 0;
+}

@@ -1,0 +1,116 @@
+"""Host-side parts of the C driver (libdnsio.so: case reader, wall-texture
+masks, file writers) against the reference's own files in the golden
+fixtures.  No GPU needed."""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+
+import fields
+import golden_util as gu
+import pyoracle
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(REPO, "dns-solvers_amd", "lib", "libdnsio.so")
+
+
+class Row(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in
+                ("t", "K", "flowrate", "massrate", "stup", "stdown", "stress", "pgrad", "force", "rho")]
+
+
+class RunCase(ctypes.Structure):
+    _fields_ = [("Rebs", ctypes.c_double), ("ub", ctypes.c_double), ("CFL", ctypes.c_double),
+                ("xdv", ctypes.c_int), ("ydv", ctypes.c_int), ("zdv", ctypes.c_int),
+                ("TSTR", ctypes.c_int), ("TEND", ctypes.c_int), ("ITER", ctypes.c_long),
+                ("forcing", ctypes.c_int), ("force", ctypes.c_double), ("precision", ctypes.c_int),
+                ("omega", ctypes.c_double), ("texture", ctypes.c_int), ("walls", ctypes.c_int),
+                ("period_x", ctypes.c_int), ("period_y", ctypes.c_int), ("slip_x", ctypes.c_int),
+                ("slip_y", ctypes.c_int), ("offset_x", ctypes.c_int), ("offset_y", ctypes.c_int),
+                ("init", ctypes.c_int), ("init_uc", ctypes.c_double), ("init_amp", ctypes.c_double),
+                ("init_seed", ctypes.c_uint64), ("init_modes", ctypes.c_int), ("outdir", ctypes.c_char * 512)]
+
+
+@pytest.fixture(scope="module")
+def lib():
+    L = ctypes.CDLL(LIB)
+    L.dnsio_format_row.argtypes = [ctypes.c_int, ctypes.POINTER(Row), ctypes.c_char_p, ctypes.c_int]
+    L.dns_case_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(RunCase), ctypes.c_char_p, ctypes.c_int]
+    L.dns_case_defaults.argtypes = [ctypes.POINTER(RunCase)]
+    L.dns_case_masks.argtypes = [ctypes.POINTER(RunCase), ctypes.c_void_p, ctypes.c_void_p]
+    L.dnsio_write_turbfld.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_void_p, ctypes.c_int]
+    L.dnsio_write_twoD.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
+    return L
+
+
+def oracle_logs(name):
+    g = gu.load(name)
+    p = gu.params(g)
+    X, Y, Z = p["X"], p["Y"], p["Z"]
+    b, t = gu.wall_mask(p)
+    o = pyoracle.Oracle(X, Y, Z, pyoracle.omega(p["Rebs"], p["ub"], Z), p["ub"], p["CFL"], TM=p["TSTR"],
+                        mask=pyoracle.wall_mask(X, Y, Z, b, t))
+    ft = gu.initial_state(p)
+    f = ft.copy()
+    logs = o.run(ft, f, p["N"])
+    last = o.logs_of(ft, p["N"])
+    rows = {k: np.append(logs[k], last[k]) for k in logs}
+    o.run(ft, f, 1, cntr0=p["N"])  # output_twoD sees the state after up+1 steps
+    return g, p, o, rows, ft
+
+
+@pytest.mark.parametrize("name", ["posts8", "smooth8"])
+def test_log_rows_byte_identical_to_reference(lib, name):
+    g, p, o, rows, _ = oracle_logs(name)
+    keys = ("K", "flowrate", "massrate", "stup", "stdown", "stress", "pgrad")
+    buf = ctypes.create_string_buffer(128)
+    for w, k in enumerate(keys):
+        out = bytearray()
+        for i in range(len(rows["t"])):
+            r = Row(*[rows[f][i] for f in ("t", "K", "flowrate", "massrate", "stup", "stdown", "stress",
+                                            "pgrad", "force", "rho")])
+            n = lib.dnsio_format_row(w, ctypes.byref(r), buf, 128)
+            out += buf.raw[:n]
+        assert bytes(out) == g["log_raw_" + k].tobytes(), k
+
+
+def test_uavg_and_twod_files_byte_identical(lib, tmp_path):
+    g, p, o, rows, ft_end = oracle_logs("posts8")
+    zd = p["Z"] + 2
+    stats = np.ascontiguousarray(g["turbfld"][-1])
+    d = str(tmp_path).encode()
+    assert lib.dnsio_write_turbfld(d, 110, stats.ctypes.data, zd) == 0
+    assert open(tmp_path / "turb-fld.000.110", "rb").read() == stats.tobytes()
+    assert open(tmp_path / "uavg-temp.dat", "rb").read() == g["raw_uavg_temp_dat"].tobytes()
+    ex, lb = o.twoD(ft_end)
+    lb = np.ascontiguousarray(lb)
+    assert lib.dnsio_write_twoD(d, lb.ctypes.data, zd, p["ub"]) == 0
+    assert open(tmp_path / "2dresults-exact.dat", "rb").read() == g["raw_2dresults_exact_dat"].tobytes()
+    assert open(tmp_path / "2dresults-lb.dat", "rb").read() == g["raw_2dresults_lb_dat"].tobytes()
+
+
+def test_case_file_and_reference_textures(lib, tmp_path):
+    cf = tmp_path / "dns.case"
+    cf.write_text("# posts case\nRebs = 60.0\nub = 0.05\nxdv = 12\nydv = 8\nzdv = 10\nTSTR = 5\nTEND = 6\n"
+                  "CFL = 0.25\nITER = 40\nPSLIP = 1\nforcing = constf\nforce = 1e-6\nprecision = fp32\n")
+    c = RunCase()
+    err = ctypes.create_string_buffer(512)
+    assert lib.dns_case_read(str(cf).encode(), ctypes.byref(c), err, 512) == -1  # defaults not applied
+    lib.dns_case_defaults(ctypes.byref(c))
+    assert lib.dns_case_read(str(cf).encode(), ctypes.byref(c), err, 512) == 0, err.value
+    assert (c.xdv, c.ydv, c.zdv, c.TSTR, c.ITER, c.texture, c.forcing, c.precision) == (12, 8, 10, 5, 40, 1, 1, 1)
+    b = np.zeros((12, 8), np.int8)
+    t = np.zeros((12, 8), np.int8)
+    lib.dns_case_masks(ctypes.byref(c), b.ctypes.data, t.ctypes.data)
+    # PSLIP defaults are the pattern of slip_posts_init.c (pinned by the
+    # golden posts runs through fields.posts_mask)
+    assert np.array_equal(b, fields.posts_mask(12, 8)) and np.array_equal(t, b)
+    c.texture = 2  # RSLIP: slip_ridges_init.c stripes j in {1,2} mod 4
+    lib.dns_case_masks(ctypes.byref(c), b.ctypes.data, t.ctypes.data)
+    assert np.array_equal(b, fields.stripes_mask(12, 8, 4, 2, offset=1))
+    bad = tmp_path / "bad.case"
+    bad.write_text("xdv = 8\nnonsense = 3\n")
+    lib.dns_case_defaults(ctypes.byref(c))
+    assert lib.dns_case_read(str(bad).encode(), ctypes.byref(c), err, 512) == -1
+    assert b"nonsense" in err.value
