@@ -116,6 +116,17 @@ def cpu_baseline(wname, budget_s=12.0):
             "sample": f"oracle/lbm_oracle.c, {X}x{Y}x{Z} slab of the workload, {n} steps, FP64, 1 thread"}
 
 
+def measured_traffic(wname):
+    """HBM bytes per launch of the step kernel from the committed rocprofv3
+    FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated against a
+    copy of known size), or None when no summary exists."""
+    path = os.path.join(HERE, "profiles", f"traffic_{wname}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        return json.load(fh)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +149,9 @@ def main():
         print(f"# fp32 done: {r32['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(args.workload)
     w = WORKLOADS[args.workload]
+    tr = measured_traffic(args.workload)
+    t64 = tr["fp64_stats"]["traffic_bytes_per_launch"] if tr else None
+    t32 = tr["fp32_stats"]["traffic_bytes_per_launch"] if tr else None
     out = {
         "metric": "MLUPS + % HBM roofline (FP32 vs FP64 D3Q19); ensemble solves/h + p99 latency",
         "value": round(r64["mlups"], 1),
@@ -156,14 +170,18 @@ def main():
                    "per_step": "fused collide-stream + per-step log reductions (encalc/wshearstr)",
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(r64["achieved_gbs"], 1), "peak": PEAK_HBM_GBS,
-                     "unit": "GB/s", "frac": round(r64["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(r64["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": t64,
                      "kernel": "step_kernel<double,false,true>", "kernel_ms": round(r64["kernel_ms"], 4),
-                     "bytes_per_node": r64["bytes_per_node"]},
+                     "bytes_per_node": r64["bytes_per_node"],
+                     "traffic_source": f"profiles/traffic_{args.workload}.json" if tr else None},
         "fp64": r64,
     }
     if r32:
-        out["fp32_shifted"] = dict(r32, roofline_frac=round(r32["achieved_gbs"] / PEAK_HBM_GBS, 4),
-                                   speedup_vs_fp64=round(r32["mlups"] / r64["mlups"], 3))
+        out["fp32_shifted"] = dict(r32, speedup_vs_fp64=round(r32["mlups"] / r64["mlups"], 3),
+                                   roofline={"bound": "hbm", "achieved": round(r32["achieved_gbs"], 1),
+                                             "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                             "frac": round(r32["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": t32,
+                                             "kernel": "step_kernel<float,false,true>"})
     if cpu:
         out["cpu_baseline"] = cpu
     print(json.dumps(out))
