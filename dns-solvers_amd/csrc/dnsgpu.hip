@@ -51,8 +51,10 @@ struct dnsgpu {
     int8_t* mbot;
     int8_t* mtop;
     uint32_t* wword;
-    int16_t* hbot;
-    int16_t* htop;
+    uint32_t* bands;  // [X*Y] solid band words hb | ht << 16
+    double* nf;       // [Z] fluid nodes per plane
+    double nfluid;    // fluid nodes in total
+    int has_bands;
     double* part;     // [kNQ][Z][nblk]
     double* gpart;    // [kNG][nblk]
     double* planes;   // [kNQ*Z + kNG] reduced sums (planes then globals)
@@ -111,6 +113,7 @@ static StepArgs step_args(dnsgpu* h, const void* src, void* dst, int nblk)
     A.mbot = h->mbot;
     A.mtop = h->mtop;
     A.wword = h->wword;
+    A.bands = h->has_bands ? h->bands : nullptr;
     A.omega = h->omega;
     A.G = &h->st->G;
     A.shift = h->shift;
@@ -139,6 +142,8 @@ static FinalizeArgs fin_args(dnsgpu* h, bool stats, LogRow* row, long long cntr,
     F.dt = h->dt;
     F.cntr = cntr;
     F.from_moments = from_moments;
+    F.nf = h->has_bands ? h->nf : nullptr;
+    F.nfluid = h->has_bands ? h->nfluid : (double)h->g.X * h->g.Y * h->g.Z;
     return F;
 }
 
@@ -163,17 +168,25 @@ static void enqueue_step_kernel(dnsgpu* h, bool stats)
     StepArgs A = step_args(h, src, dst, h->step_nblk);
     dim3 grid(h->step_nblk), block(kWave * h->step_w);
     const size_t lds = stats ? step_lds_bytes(h->g.Z, h->step_w) : 0;
-    if (h->poststream) {
-        if (stats)
-            hipLaunchKernelGGL((step_kernel<T, true, true>), grid, block, lds, h->stream, A);
-        else
-            hipLaunchKernelGGL((step_kernel<T, true, false>), grid, block, lds, h->stream, A);
+#define STEP_VARIANT(F, S, B) hipLaunchKernelGGL((step_kernel<T, F, S, B>), grid, block, lds, h->stream, A)
+    if (h->has_bands) {
+        if (h->poststream) {
+            if (stats) STEP_VARIANT(true, true, true);
+            else STEP_VARIANT(true, false, true);
+        } else {
+            if (stats) STEP_VARIANT(false, true, true);
+            else STEP_VARIANT(false, false, true);
+        }
     } else {
-        if (stats)
-            hipLaunchKernelGGL((step_kernel<T, false, true>), grid, block, lds, h->stream, A);
-        else
-            hipLaunchKernelGGL((step_kernel<T, false, false>), grid, block, lds, h->stream, A);
+        if (h->poststream) {
+            if (stats) STEP_VARIANT(true, true, false);
+            else STEP_VARIANT(true, false, false);
+        } else {
+            if (stats) STEP_VARIANT(false, true, false);
+            else STEP_VARIANT(false, false, false);
+        }
     }
+#undef STEP_VARIANT
 }
 
 template <typename T>
@@ -314,8 +327,8 @@ extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
         {(void**)&h->mbot, xy},
         {(void**)&h->mtop, xy},
         {(void**)&h->wword, xy * 4},
-        {(void**)&h->hbot, xy * 2},
-        {(void**)&h->htop, xy * 2},
+        {(void**)&h->bands, xy * 4},
+        {(void**)&h->nf, sizeof(double) * g.Z},
         {(void**)&h->part, sizeof(double) * kNQ * g.Z * (size_t)std::max(h->nblk, h->step_nblk_max)},
         {(void**)&h->gpart, sizeof(double) * kNG * (size_t)std::max(h->nblk, h->step_nblk_max)},
         {(void**)&h->planes, sizeof(double) * (kNQ * g.Z + kNG)},
@@ -332,8 +345,7 @@ extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
     h->ring_cap = 1024;
     if (hipMemsetAsync(h->mbot, -1, xy, h->stream) != hipSuccess ||
         hipMemsetAsync(h->mtop, -1, xy, h->stream) != hipSuccess ||
-        hipMemsetAsync(h->hbot, 0, xy * 2, h->stream) != hipSuccess ||
-        hipMemsetAsync(h->htop, 0, xy * 2, h->stream) != hipSuccess ||
+        hipMemsetAsync(h->bands, 0, xy * 4, h->stream) != hipSuccess ||
         hipMemsetAsync(h->st, 0, sizeof(DevState), h->stream) != hipSuccess ||
         hipMemsetAsync(h->shift, 0, sizeof(double) * 3 * g.Z, h->stream) != hipSuccess)
         return fail(set_err(DNSGPU_EHIP, "initial memset failed"));
@@ -352,7 +364,7 @@ extern "C" void dnsgpu_destroy(dnsgpu* h)
 {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    void* ptrs[] = {h->buf[0], h->buf[1], h->mbot, h->mtop, h->wword, h->hbot, h->htop, h->part,
+    void* ptrs[] = {h->buf[0], h->buf[1], h->mbot, h->mtop, h->wword, h->bands, h->nf, h->part,
                     h->gpart,  h->planes, h->shift, h->means, h->st, h->ring, h->stage};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
@@ -372,9 +384,12 @@ extern "C" int dnsgpu_set_walls(dnsgpu* h, const int8_t* bottom, const int8_t* t
     int8_t* dst[2] = {h->mbot, h->mtop};
     for (int w = 0; w < 2; w++) {
         if (src[w]) {
-            for (size_t i = 0; i < xy; i++)
+            for (size_t i = 0; i < xy; i++) {
                 if (src[w][i] != -1 && src[w][i] != -2)
                     return set_err(DNSGPU_EARG, "wall code %d at %zu (must be -1 or -2)", src[w][i], i);
+                if (h->has_bands && src[w][i] == -2)
+                    return set_err(DNSGPU_EARG, "slip wall nodes cannot be combined with solid bands");
+            }
             HIPCHK(hipMemcpyAsync(dst[w], src[w], xy, hipMemcpyHostToDevice, h->stream));
         } else {
             HIPCHK(hipMemsetAsync(dst[w], -1, xy, h->stream));
@@ -389,14 +404,37 @@ extern "C" int dnsgpu_set_walls(dnsgpu* h, const int8_t* bottom, const int8_t* t
 extern "C" int dnsgpu_set_solid_bands(dnsgpu* h, const int16_t* bottom, const int16_t* top)
 {
     if (!h) return set_err(DNSGPU_EARG, "null handle");
-    const size_t xy = (size_t)h->g.X * h->g.Y;
+    const int X = h->g.X, Y = h->g.Y, Z = h->g.Z;
+    const size_t xy = (size_t)X * Y;
+    std::vector<uint32_t> bw(xy, 0u);
+    std::vector<double> nf(Z, 0.0);
     bool any = false;
-    for (int w = 0; w < 2; w++) {
-        const int16_t* s = w ? top : bottom;
-        if (s)
-            for (size_t i = 0; i < xy; i++) any |= s[i] != 0;
+    for (size_t i = 0; i < xy; i++) {
+        const int hb = bottom ? bottom[i] : 0, ht = top ? top[i] : 0;
+        if (hb < 0 || ht < 0 || hb + ht > Z - 2)
+            return set_err(DNSGPU_EARG, "solid band %d + %d at column %zu leaves fewer than 2 fluid rows of %d", hb,
+                           ht, i, Z);
+        bw[i] = (uint32_t)hb | ((uint32_t)ht << 16);
+        any |= (hb | ht) != 0;
+        for (int z = hb; z < Z - ht; z++) nf[z] += 1.0;
     }
-    if (any) return set_err(DNSGPU_EARG, "solid bands inside the fluid rows are not supported by this build");
+    if (any) {
+        // bands are no-slip solids: the walls must be no-slip too
+        std::vector<int8_t> m(xy);
+        for (int w = 0; w < 2; w++) {
+            HIPCHK(hipMemcpyAsync(m.data(), w ? h->mtop : h->mbot, xy, hipMemcpyDeviceToHost, h->stream));
+            HIPCHK(hipStreamSynchronize(h->stream));
+            for (size_t i = 0; i < xy; i++)
+                if (m[i] == -2) return set_err(DNSGPU_EARG, "solid bands need no-slip walls (slip node at %zu)", i);
+        }
+    }
+    double total = 0.0;
+    for (int z = 0; z < Z; z++) total += nf[z];
+    HIPCHK(hipMemcpyAsync(h->bands, bw.data(), xy * 4, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipMemcpyAsync(h->nf, nf.data(), sizeof(double) * Z, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->has_bands = any ? 1 : 0;
+    h->nfluid = total;
     return 0;
 }
 
@@ -510,6 +548,7 @@ extern "C" int dnsgpu_save_aos(dnsgpu* h, int x0, int nx, double* ftemp, double*
     A.other = h->buf[1 - h->cur];
     A.mbot = h->mbot;
     A.mtop = h->mtop;
+    A.bands = h->has_bands ? h->bands : nullptr;
     A.local = h->poststream;
     A.x0 = x0;
     A.nx = nx;
@@ -581,6 +620,7 @@ static int launch_stats_pass(dnsgpu* h, int pass)
     A.cur = h->buf[h->cur];
     A.mbot = h->mbot;
     A.mtop = h->mtop;
+    A.bands = h->has_bands ? h->bands : nullptr;
     A.local = h->poststream;
     A.pass = pass;
     A.means = h->means;
@@ -639,6 +679,7 @@ extern "C" int dnsgpu_fields(dnsgpu* h, int x0, int nx, double* vel, double* den
     A.cur = h->buf[h->cur];
     A.mbot = h->mbot;
     A.mtop = h->mtop;
+    A.bands = h->has_bands ? h->bands : nullptr;
     A.local = h->poststream;
     A.x0 = x0;
     A.nx = nx;

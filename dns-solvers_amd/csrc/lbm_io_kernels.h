@@ -22,13 +22,13 @@ __device__ __forceinline__ float from_f<float>(double f, int a) { return (float)
 // Populations of the current post-stream state of node (col, z) as f (FP64).
 template <typename T>
 __device__ __forceinline__ void state_f(const T* src, const Column& c, int z, const int8_t* mbot,
-                                        const int8_t* mtop, bool local, double (&f)[Q])
+                                        const int8_t* mtop, const uint32_t* bands, bool local, double (&f)[Q])
 {
     T v[Q];
     if (local)
         load_local19(src, c, z, v);
     else
-        pull19(src, c, z, mbot, mtop, v);
+        pull19(src, c, z, mbot, mtop, bands, v);
 #pragma unroll
     for (int a = 0; a < Q; a++) f[a] = Store<T>::to_f(v[a], a);
 }
@@ -43,6 +43,7 @@ struct AosArgs {
     void* other_w;
     const int8_t* mbot;
     const int8_t* mtop;
+    const uint32_t* bands;
     int local;
     int x0, nx;
     double* ftemp;  // staging [nx][Y][Z+2][19]
@@ -67,7 +68,9 @@ __global__ __launch_bounds__(256) void save_aos_kernel(AosArgs A)
         const int x = A.x0 + (int)(cl / Y);
         double* ft = A.ftemp + i * Q;
         double* fo = A.f + i * Q;
-        if (k == 0 || k == Zd - 1) {
+        if (k == 0 || k == Zd - 1 || (A.bands && band_solid(A.bands[x * Y + y], k - 1, Z))) {
+            // wall rows and solid band nodes: zeros, as in a reference
+            // checkpoint (the solver never writes solid nodes)
 #pragma unroll
             for (int a = 0; a < Q; a++) {
                 ft[a] = 0.0;
@@ -78,7 +81,7 @@ __global__ __launch_bounds__(256) void save_aos_kernel(AosArgs A)
         const int z = k - 1;
         const Column c = make_column_lane(x * Y + y, X, Y, Z);
         double fs[Q];
-        state_f<T>((const T*)A.cur, c, z, A.mbot, A.mtop, A.local != 0, fs);
+        state_f<T>((const T*)A.cur, c, z, A.mbot, A.mtop, A.bands, A.local != 0, fs);
         const T* ob = (const T*)(A.local ? A.other : A.cur);
 #pragma unroll
         for (int a = 0; a < Q; a++) {
@@ -119,6 +122,7 @@ struct FieldArgs {
     const void* cur;
     const int8_t* mbot;
     const int8_t* mtop;
+    const uint32_t* bands;
     int local;
     int x0, nx;
     double* vel;  // [nx][Y][Z+2][3]
@@ -139,10 +143,10 @@ __global__ __launch_bounds__(256) void fields_kernel(FieldArgs A)
         const int y = (int)(cl % Y);
         const int x = A.x0 + (int)(cl / Y);
         double ux = 0., uy = 0., uz = 0., dens = 0.;
-        if (k >= 1 && k <= Z) {
+        if (k >= 1 && k <= Z && !(A.bands && band_solid(A.bands[x * Y + y], k - 1, Z))) {
             const Column c = make_column_lane(x * Y + y, X, Y, Z);
             double f[Q];
-            state_f<T>((const T*)A.cur, c, k - 1, A.mbot, A.mtop, A.local != 0, f);
+            state_f<T>((const T*)A.cur, c, k - 1, A.mbot, A.mtop, A.bands, A.local != 0, f);
             moments_fp64(f, ux, uy, uz, dens);
         }
         if (dens > 0.) {
@@ -169,6 +173,7 @@ struct StatArgs {
     const void* cur;
     const int8_t* mbot;
     const int8_t* mtop;
+    const uint32_t* bands;
     int local;
     int pass;             // 1: sums of u,v,w,rho ; 2: sums of squared fluctuations
     const double* means;  // [4][Z] UAVG,VAVG,WAVG,DAVG (pass 2)
@@ -201,9 +206,10 @@ __global__ __launch_bounds__(kWave* kMaxWaves) void stats_kernel(StatArgs A)
     for (int grp = blockIdx.x; grp < A.g.ngroups; grp += gridDim.x) {
         const int col = __builtin_amdgcn_readfirstlane(grp * A.g.cpb + slot);
         if (col >= A.g.ncols || !zvalid) continue;
+        if (A.bands && band_solid(A.bands[col], z, Z)) continue;  // statistics.c sums fluid nodes
         const Column c = make_column(col, X, Y, Z);
         double f[Q];
-        state_f<T>((const T*)A.cur, c, z, A.mbot, A.mtop, A.local != 0, f);
+        state_f<T>((const T*)A.cur, c, z, A.mbot, A.mtop, A.bands, A.local != 0, f);
         double u, v, w, dens;
         moments_fp64(f, u, v, w, dens);
         if (A.pass == 1) {

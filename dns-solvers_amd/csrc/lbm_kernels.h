@@ -60,6 +60,7 @@ struct StepArgs {
     const int8_t* mbot;     // [X*Y] bottom wall codes (-1 / -2)
     const int8_t* mtop;     // [X*Y] top wall codes
     const uint32_t* wword;  // [X*Y] per-column wall words
+    const uint32_t* bands;  // [X*Y] solid bands hb | ht << 16 (null: flat walls only)
     double omega;
     const double* G;      // force used by this step (device scalar)
     const double* shift;  // [3][Z] plane-mean shifts for the one-pass variances
@@ -154,12 +155,29 @@ __device__ __forceinline__ uint32_t load_uniform_u32(const uint32_t* p)
     return *(const __attribute__((address_space(4))) uint32_t*)p;
 }
 
+// Solid bands inside the fluid rows (riblets): column (x, y) is solid for
+// rows z < hb and z >= Z - ht, packed as hb | ht << 16.  No reference
+// counterpart (the reference has solids only in the wall rows); the rule
+// is halfway bounce-back whenever the upstream node is solid, which is the
+// reference's wall rule when the solid is a no-slip wall row.
+__device__ __forceinline__ bool band_solid(uint32_t bw, int zs, int Z)
+{
+    const int hb = (int)(bw & 0xFFFFu), ht = (int)(bw >> 16);
+    return (zs >= 0 && zs < hb) || (zs < Z && zs >= Z - ht);
+}
+
+__device__ __forceinline__ uint32_t band_word_lane(const uint32_t* bands, int x, int y, int Y)
+{
+    return bands ? bands[x * Y + y] : 0u;
+}
+
 // Gather the post-stream populations of node (column c, row z) from the
 // post-collision array src (pull), wall rule included.  Generic per-thread
 // form used by the cadence kernels.
 template <typename T>
 __device__ __forceinline__ void pull19(const T* __restrict__ src, const Column& c, int z,
-                                       const int8_t* __restrict__ mbot, const int8_t* __restrict__ mtop, T (&v)[Q])
+                                       const int8_t* __restrict__ mbot, const int8_t* __restrict__ mtop,
+                                       const uint32_t* __restrict__ bands, T (&v)[Q])
 {
     const int Z = c.Z, Y = c.Y;
     static_for<0, Q>([&](auto ac) {
@@ -197,6 +215,14 @@ __device__ __forceinline__ void pull19(const T* __restrict__ src, const Column& 
                 else
                     v[a] = src[c.cb + (long long)OPP[a] * Z + z];
             }
+        });
+    }
+    if (bands) {
+        static_for<1, Q>([&](auto ac) {
+            constexpr int a = decltype(ac)::value;
+            const int sx = EX[a] == 1 ? c.xm : (EX[a] == -1 ? c.xp : c.x);
+            const int sy = EY[a] == 1 ? c.ym : (EY[a] == -1 ? c.yp : c.y);
+            if (band_solid(bands[sx * Y + sy], z - EZ[a], Z)) v[a] = src[c.cb + (long long)OPP[a] * Z + z];
         });
     }
 }
@@ -287,7 +313,7 @@ __host__ __device__ inline size_t step_lds_bytes(int Z, int W) { return sizeof(d
 // term, whose x-moment is exactly G, and streaming is a permutation except
 // that a halfway bounce-back reverses e_x:
 //     sum f_next = sum f,  sum e_x f_next = sum (j_x + G) - 2 sum_{BB} e_x g
-template <typename T, bool FIRST, bool STATS>
+template <typename T, bool FIRST, bool STATS, bool BANDS>
 __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs A)
 {
     extern __shared__ double lds[];  // STATS: [W][kNQ][Z] plane sums, then [3][Z] shifts
@@ -314,6 +340,20 @@ __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs A)
         const int col = __builtin_amdgcn_readfirstlane(it);
         const Column c = make_column(col, X, Y, Z);
         const uint32_t ww = load_uniform_u32(A.wword + col);
+        // solid bands of the 3x3 neighbourhood, bw[ex+1][ey+1] = column (x - ex, y - ey)
+        uint32_t bw[3][3];
+        int maxhb = 0, maxht = 0;
+        if constexpr (BANDS) {
+            const int sx[3] = {c.xp, c.x, c.xm}, sy[3] = {c.yp, c.y, c.ym};
+#pragma unroll
+            for (int i = 0; i < 3; i++)
+#pragma unroll
+                for (int j = 0; j < 3; j++) {
+                    bw[i][j] = load_uniform_u32(A.bands + sx[i] * Y + sy[j]);
+                    maxhb = max(maxhb, (int)(bw[i][j] & 0xFFFFu));
+                    maxht = max(maxht, (int)(bw[i][j] >> 16));
+                }
+        }
         for (int ch = 0; ch < nzc; ch++) {
             const int z = ch * kWave + lane;
             if (z >= Z) break;
@@ -324,8 +364,23 @@ __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs A)
             const uint32_t zoff_dn = (uint32_t)(z == Z - 1 ? Z - 2 : z) * (uint32_t)sizeof(T);
             T v[Q];
             gather_wave<T, FIRST>(src, c, zoff, zoff_up, zoff_dn, bot_lane, top_lane, z, ww, v);
+            // chunk touches a band of the neighbourhood (upstream rows z-1..z+1)
+            bool slow = false, solid_own = false;
+            if constexpr (BANDS) {
+                const int z0 = ch * kWave, z1 = min(z0 + kWave, Z);
+                slow = (maxhb > 0 && z0 - 1 < maxhb) || (maxht > 0 && z1 >= Z - maxht);
+                solid_own = band_solid(bw[1][1], z, Z);
+                if (slow && !FIRST) {
+                    static_for<1, Q>([&](auto ac) {
+                        constexpr int a = decltype(ac)::value;
+                        if (band_solid(bw[EX[a] + 1][EY[a] + 1], z - EZ[a], Z))
+                            v[a] = src[c.cb + (long long)OPP[a] * Z + z];
+                    });
+                }
+            }
 
             double mass_node, jx_raw, g11, g12, g13, g14;
+            double gx[Q];  // post-collision values of the e_x != 0 directions (budget)
             if constexpr (std::is_same<T, double>::value) {
                 double ux, uy, uz, dens;
                 moments_fp64(v, ux, uy, uz, dens);
@@ -334,52 +389,70 @@ __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs A)
                 jx_raw = ux;
                 mass_node = dens - 1.0;
                 if constexpr (STATS) {
-                    const double du = u0 - shl[z], dv = u1 - shl[Z + z], dw = u2 - shl[2 * Z + z];
-                    mine[0 * Z + z] += du;
-                    mine[1 * Z + z] += dv;
-                    mine[2 * Z + z] += dw;
-                    mine[3 * Z + z] += mass_node;
-                    mine[4 * Z + z] += (du * du + dv * dv) + dw * dw;
+                    if (!solid_own) {
+                        const double du = u0 - shl[z], dv = u1 - shl[Z + z], dw = u2 - shl[2 * Z + z];
+                        mine[0 * Z + z] += du;
+                        mine[1 * Z + z] += dv;
+                        mine[2 * Z + z] += dw;
+                        mine[3 * Z + z] += mass_node;
+                        mine[4 * Z + z] += (du * du + dv * dv) + dw * dw;
+                    }
                 }
-                g11 = v[11];
-                g12 = v[12];
-                g13 = v[13];
-                g14 = v[14];
+#pragma unroll
+                for (int a = 0; a < Q; a++) gx[a] = v[a];
             } else {
                 float drho, u0, u1, u2, jx;
                 collide_fp32(v, (float)A.omega, (float)G, drho, u0, u1, u2, jx);
                 jx_raw = (double)jx;
                 mass_node = (double)drho;
                 if constexpr (STATS) {
-                    const double du = (double)u0 - shl[z], dv = (double)u1 - shl[Z + z],
-                                 dw = (double)u2 - shl[2 * Z + z];
-                    mine[0 * Z + z] += du;
-                    mine[1 * Z + z] += dv;
-                    mine[2 * Z + z] += dw;
-                    mine[3 * Z + z] += mass_node;
-                    mine[4 * Z + z] += (du * du + dv * dv) + dw * dw;
+                    if (!solid_own) {
+                        const double du = (double)u0 - shl[z], dv = (double)u1 - shl[Z + z],
+                                     dw = (double)u2 - shl[2 * Z + z];
+                        mine[0 * Z + z] += du;
+                        mine[1 * Z + z] += dv;
+                        mine[2 * Z + z] += dw;
+                        mine[3 * Z + z] += mass_node;
+                        mine[4 * Z + z] += (du * du + dv * dv) + dw * dw;
+                    }
                 }
-                g11 = WD[11] + (double)v[11];
-                g12 = WD[12] + (double)v[12];
-                g13 = WD[13] + (double)v[13];
-                g14 = WD[14] + (double)v[14];
+#pragma unroll
+                for (int a = 0; a < Q; a++) gx[a] = WD[a] + (double)v[a];
             }
+            g11 = gx[11];
+            g12 = gx[12];
+            g13 = gx[13];
+            g14 = gx[14];
             double mom_node = jx_raw + G;
-            // bounce-back reverses e_x of the populations pushed into a
-            // no-slip wall node from a no-slip wall column (xcomputations.c:370)
-            if (bot_lane)
-                mom_node = mom_node - (((ww >> 10) & 1u) ? 2.0 * g12 : 0.0) + (((ww >> 11) & 1u) ? 2.0 * g13 : 0.0);
-            if (top_lane)
-                mom_node = mom_node - (((ww >> 12) & 1u) ? 2.0 * g11 : 0.0) + (((ww >> 13) & 1u) ? 2.0 * g14 : 0.0);
-            if constexpr (STATS) gjx += jx_raw;
-            gmass += mass_node;
-            gmom += mom_node;
-
-            static_for<0, Q>([&](auto ac) {
-                constexpr int a = decltype(ac)::value;
-                const long long e = c.cb + (long long)a * Z;
-                *(T*)(d8 + e * (long long)sizeof(T) + zoff) = v[a];
-            });
+            if (BANDS && slow) {
+                // every push of an e_x != 0 population into a solid node (band
+                // or no-slip wall row) bounces back and reverses e_x
+                static_for<1, Q>([&](auto ac) {
+                    constexpr int a = decltype(ac)::value;
+                    if constexpr (EX[a] != 0) {
+                        const int zt = z + EZ[a];
+                        const bool solid_t = zt < 0 || zt >= Z || band_solid(bw[1 - EX[a]][1 - EY[a]], zt, Z);
+                        if (solid_t) mom_node -= 2.0 * (double)EX[a] * gx[a];
+                    }
+                });
+            } else {
+                // bounce-back reverses e_x of the populations pushed into a
+                // no-slip wall node from a no-slip wall column (xcomputations.c:370)
+                if (bot_lane)
+                    mom_node = mom_node - (((ww >> 10) & 1u) ? 2.0 * g12 : 0.0) + (((ww >> 11) & 1u) ? 2.0 * g13 : 0.0);
+                if (top_lane)
+                    mom_node = mom_node - (((ww >> 12) & 1u) ? 2.0 * g11 : 0.0) + (((ww >> 13) & 1u) ? 2.0 * g14 : 0.0);
+            }
+            if (!solid_own) {
+                if constexpr (STATS) gjx += jx_raw;
+                gmass += mass_node;
+                gmom += mom_node;
+                static_for<0, Q>([&](auto ac) {
+                    constexpr int a = decltype(ac)::value;
+                    const long long e = c.cb + (long long)a * Z;
+                    *(T*)(d8 + e * (long long)sizeof(T) + zoff) = v[a];
+                });
+            }
         }
     }
 
@@ -497,11 +570,12 @@ __global__ __launch_bounds__(kWave* kMaxWaves) void moments_kernel(StepArgs A)
         const int col = __builtin_amdgcn_readfirstlane(grp * A.g.cpb + slot);
         if (col >= A.g.ncols || !zvalid) continue;
         const Column c = make_column(col, X, Y, Z);
+        if (A.bands && band_solid(A.bands[col], z, Z)) continue;
         T v[Q];
         if constexpr (LOCAL)
             load_local19(src, c, z, v);
         else
-            pull19(src, c, z, A.mbot, A.mtop, v);
+            pull19(src, c, z, A.mbot, A.mtop, A.bands, v);
         double f[Q];
 #pragma unroll
         for (int a = 0; a < Q; a++) f[a] = Store<T>::to_f(v[a], a);
@@ -568,6 +642,8 @@ struct FinalizeArgs {
     double ub, F, mu, TM, dt;
     long long cntr;
     int from_moments;       // 1: globals came from moments_kernel (initial state)
+    const double* nf;       // [Z] fluid nodes per plane (null: X*Y, flat walls)
+    double nfluid;          // fluid nodes in total
 };
 
 // Per-step finisher (encalc.c:29-138 and wshearstr.c:26-154 restated on
@@ -584,16 +660,22 @@ __global__ __launch_bounds__(1024) void finalize_kernel(FinalizeArgs A)
     const bool have_planes = A.planes != nullptr;
     double uz = 0.0, kz = 0.0;
     if (have_planes && t < Z) {
-        const double du = A.planes[0 * Z + t] / Nxy;
-        const double dv = A.planes[1 * Z + t] / Nxy;
-        const double dw = A.planes[2 * Z + t] / Nxy;
-        uz = A.shift[t] + du;
-        kz = A.planes[4 * Z + t] - Nxy * ((du * du + dv * dv) + dw * dw);
+        // plane means over the whole plane with solid nodes counting as
+        // u = 0 (encalc.c:54-59 divides by the full plane); sums run over the
+        // nf fluid nodes, shifted by c:  mean = (nf c + S1) / Nxy and
+        // sum_f |u - mean|^2 = S2 - 2 d.S1 + nf |d|^2 with d = mean - c
+        const double nf = A.nf ? A.nf[t] : Nxy;
+        const double s1u = A.planes[0 * Z + t], s1v = A.planes[1 * Z + t], s1w = A.planes[2 * Z + t];
+        const double cu = A.shift[t], cv = A.shift[Z + t], cw = A.shift[2 * Z + t];
+        const double mu_ = (nf * cu + s1u) / Nxy, mv = (nf * cv + s1v) / Nxy, mw = (nf * cw + s1w) / Nxy;
+        const double du = mu_ - cu, dv = mv - cv, dw = mw - cw;
+        uz = mu_;
+        kz = A.planes[4 * Z + t] - 2.0 * ((du * s1u + dv * s1v) + dw * s1w) + nf * ((du * du + dv * dv) + dw * dw);
         uav[t] = uz;
-        dav[t] = 1.0 + A.planes[3 * Z + t] / Nxy;
-        A.shift[t] = uz;
-        A.shift[Z + t] = A.shift[Z + t] + dv;
-        A.shift[2 * Z + t] = A.shift[2 * Z + t] + dw;
+        dav[t] = (nf + A.planes[3 * Z + t]) / Nxy;
+        A.shift[t] = mu_;
+        A.shift[Z + t] = mv;
+        A.shift[2 * Z + t] = mw;
     }
     sU[t] = uz;
     sK[t] = kz;
@@ -610,8 +692,8 @@ __global__ __launch_bounds__(1024) void finalize_kernel(FinalizeArgs A)
         const double jx = A.globals[0];
         if (A.row && have_planes) {
             double srho = 0.0;
-            for (int z = 0; z < Z; z++) srho += dav[z] - 1.0;
-            const double denst = 1.0 + srho / (double)Z;
+            for (int z = 0; z < Z; z++) srho += A.planes[3 * Z + z];
+            const double denst = (A.nfluid + srho) / N;
             const double momentum = jx / N;
             const double K = 0.5 * sK[0] / N;
             const double U = sU[0] / (double)Z;
@@ -637,10 +719,10 @@ __global__ __launch_bounds__(1024) void finalize_kernel(FinalizeArgs A)
             if (A.from_moments) {
                 double srho = 0.0;
                 for (int z = 0; z < Z; z++) srho += A.planes[3 * Z + z];
-                rho_next = 1.0 + srho / N;
+                rho_next = (A.nfluid + srho) / N;
                 mom_next = jx / N;
             } else {
-                rho_next = 1.0 + A.globals[1] / N;
+                rho_next = (A.nfluid + A.globals[1]) / N;
                 mom_next = A.globals[2] / N;
             }
             A.st->G = rho_next * A.ub - mom_next;

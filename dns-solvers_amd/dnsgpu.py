@@ -156,6 +156,22 @@ class Channel:
             b.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)) if b is not None else None,
             t.ctypes.data_as(ctypes.POINTER(ctypes.c_int8)) if t is not None else None))
 
+    def set_solid_bands(self, bottom=None, top=None):
+        """Solid rows above the bottom wall / below the top wall per column,
+        int16 [xdv][ydv] (riblets); None = no band."""
+        def p(m):
+            if m is None:
+                return None
+            m = np.ascontiguousarray(m, dtype=np.int16)
+            assert m.shape == (self.X, self.Y)
+            return m
+        b, t = p(bottom), p(top)
+        self._bands = (b, t)
+        _chk(load().dnsgpu_set_solid_bands(
+            self.h,
+            b.ctypes.data_as(ctypes.POINTER(ctypes.c_int16)) if b is not None else None,
+            t.ctypes.data_as(ctypes.POINTER(ctypes.c_int16)) if t is not None else None))
+
     def init_rest(self):
         _chk(load().dnsgpu_init_rest(self.h))
 

@@ -2,6 +2,7 @@
 #include "dnscase.h"
 
 #include <ctype.h>
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -32,6 +33,8 @@ void dns_case_defaults(dns_run_case *c)
     c->init = 0;
     c->init_seed = 1;
     c->init_modes = 4;
+    c->riblet_spacing = 16;
+    c->riblet_height = 8;
     strcpy(c->outdir, ".");
 }
 
@@ -56,6 +59,7 @@ static int set_key(dns_run_case *c, const char *k, const char *v, char *err, int
         return 0;                             \
     }
     DBL(Rebs) DBL(ub) DBL(CFL) DBL(force) DBL(omega) DBL(init_uc) DBL(init_amp)
+    DBL(riblet_spacing) DBL(riblet_height) DBL(riblet_offset)
     INT(xdv) INT(ydv) INT(zdv) INT(TSTR) INT(TEND) INT(period_x) INT(period_y) INT(slip_x) INT(slip_y)
     INT(offset_x) INT(offset_y) INT(init_modes)
 #undef DBL
@@ -92,6 +96,7 @@ static int set_key(dns_run_case *c, const char *k, const char *v, char *err, int
         if (!strcmp(v, "smooth")) c->texture = 0;
         else if (!strcmp(v, "posts")) c->texture = 1;
         else if (!strcmp(v, "stripes")) c->texture = 2;
+        else if (!strcmp(v, "riblets")) c->texture = 3;
         else goto bad;
         return 0;
     }
@@ -157,6 +162,10 @@ int dns_case_read(const char *path, dns_run_case *c, char *err, int errlen)
         snprintf(err, errlen, "CFL %g gives no whole step per time unit", c->CFL);
         return -1;
     }
+    if (c->texture == 3 && (c->riblet_spacing <= 0 || c->riblet_height < 0)) {
+        snprintf(err, errlen, "riblet spacing must be > 0 and height >= 0");
+        return -1;
+    }
     if (c->texture && (c->period_x < 1 || c->period_y < 1)) {
         snprintf(err, errlen, "texture periods must be >= 1");
         return -1;
@@ -176,8 +185,27 @@ void dns_case_masks(const dns_run_case *c, int8_t *bottom, int8_t *top)
                 slip = modp(x - c->offset_x, c->period_x) < c->slip_x && modp(y - c->offset_y, c->period_y) < c->slip_y;
             else if (c->texture == 2)
                 slip = modp(y - c->offset_y, c->period_y) < c->slip_y;
-            const int8_t code = slip ? -2 : -1;
+            const int8_t code = slip ? -2 : -1; /* riblet walls are no-slip */
             bottom[x * Y + y] = (c->walls == 0 || c->walls == 1) ? code : -1;
             top[x * Y + y] = (c->walls == 0 || c->walls == 2) ? code : -1;
         }
+}
+
+void dns_case_bands(const dns_run_case *c, int16_t *bottom, int16_t *top)
+{
+    const int X = c->xdv, Y = c->ydv;
+    for (int y = 0; y < Y; y++) {
+        int16_t hb = 0;
+        if (c->texture == 3) {
+            const double yc = y + 0.5 - c->riblet_offset;
+            double ph = fmod(yc, c->riblet_spacing) / c->riblet_spacing;
+            if (ph < 0) ph += 1.0;
+            const double H = c->riblet_height * (1.0 - fabs(2.0 * ph - 1.0));
+            hb = H > 0.5 ? (int16_t)ceil(H - 0.5) : 0;
+        }
+        for (int x = 0; x < X; x++) {
+            bottom[x * Y + y] = (c->walls == 0 || c->walls == 1) ? hb : 0;
+            top[x * Y + y] = (c->walls == 0 || c->walls == 2) ? hb : 0;
+        }
+    }
 }

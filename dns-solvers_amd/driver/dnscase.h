@@ -10,12 +10,12 @@
  *   force      = <double>             body force for constf
  *   precision  = fp64 | fp32          (default fp64)
  *   omega      = <double>             1/tau; default from Rebs, ub, zdv
- *   texture    = smooth | posts | stripes   (PSLIP / RSLIP set posts / stripes)
+ *   texture    = smooth | posts | stripes | riblets  (PSLIP / RSLIP set posts / stripes)
+ *   riblet_spacing, riblet_height, riblet_offset   (triangular riblets, cells)
  *   period_x, period_y, slip_x, slip_y, offset_x, offset_y   (texture geometry)
  *   walls      = both | bottom | top  (which walls carry the texture)
  *   init       = rest | poiseuille | turbulent   (for -s; rest is init.c)
  *   init_uc, init_amp, init_seed, init_modes
- *   log_every  = 1                    (rows per step written to the logs)
  *   outdir     = .                    (where every output file goes)
  */
 #ifndef DNSCASE_H
@@ -34,7 +34,8 @@ typedef struct dns_run_case {
     double force;
     int precision; /* 0 fp64, 1 fp32 */
     double omega;  /* 0 => derived */
-    int texture;   /* 0 smooth, 1 posts, 2 stripes */
+    int texture;   /* 0 smooth, 1 posts, 2 stripes, 3 riblets */
+    double riblet_spacing, riblet_height, riblet_offset;
     int walls;     /* 0 both, 1 bottom, 2 top */
     int period_x, period_y, slip_x, slip_y, offset_x, offset_y;
     int init;      /* 0 rest, 1 poiseuille, 2 turbulent */
@@ -58,5 +59,11 @@ int dns_case_read(const char *path, dns_run_case *c, char *err, int errlen);
  *           < slip_y; the reference's RSLIP pattern (slip_ridges_init.c:11-27)
  *           is period 4, slip 2, offset 1. */
 void dns_case_masks(const dns_run_case *c, int8_t *bottom, int8_t *top);
+
+/* Solid bands [xdv][ydv] of triangular riblets (texture 3): the cell
+ * (y, z) is solid when its centre z + 1/2 lies below the triangle of
+ * spanwise period riblet_spacing and tip height riblet_height at y + 1/2.
+ * Streamwise-invariant.  Zero for every other texture. */
+void dns_case_bands(const dns_run_case *c, int16_t *bottom, int16_t *top);
 
 #endif

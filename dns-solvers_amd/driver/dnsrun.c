@@ -225,6 +225,8 @@ static void write_report(const dns_run_case *c, double omega, double seconds, do
     fprintf(tm, "Executed File's Name: %s\n", __FILE__);
     fprintf(tm, "Compilation Option: xcut\n");
     if (c->texture == 2) fprintf(tm, "slip option: RSLIP\n");
+    if (c->texture == 3)
+        fprintf(tm, "texture: riblets spacing %g height %g\n", c->riblet_spacing, c->riblet_height);
     if (c->texture == 1) fprintf(tm, "slip option: PSLIP\n");
     fprintf(tm, "Total computational time is: %f s\n", seconds);
     fprintf(tm, "Unit non-dimensional time is: %f iterations\n", utime);
@@ -288,6 +290,15 @@ int main(int argc, char *argv[])
     API(dnsgpu_set_walls(h, mb, mt));
     free(mb);
     free(mt);
+    if (c.texture == 3) {
+        int16_t *hb = malloc(sizeof(int16_t) * (size_t)c.xdv * c.ydv);
+        int16_t *ht = malloc(sizeof(int16_t) * (size_t)c.xdv * c.ydv);
+        if (!hb || !ht) return 1;
+        dns_case_bands(&c, hb, ht);
+        API(dnsgpu_set_solid_bands(h, hb, ht));
+        free(hb);
+        free(ht);
+    }
 
     const int TM = c.TSTR;
     if (ch == 's') {

@@ -142,6 +142,10 @@ int oracle_wshearstr(const oracle_case *c, const double *ft, long cntr, oracle_l
                 const double *p = ft + POP(c, i, j, 1 + q, 0);
                 double u = 0.;
                 dens = 0.;
+                if (mask_at(c, i, j, 1 + q) < 0) { /* solid inside the fluid rows: u = 0, rho = 0 */
+                    ux[q] = 0.;
+                    continue;
+                }
                 for (int a = 0; a < 19; a++) {
                     dens += p[a];
                     u += DEX[a] * p[a];
@@ -163,6 +167,10 @@ int oracle_wshearstr(const oracle_case *c, const double *ft, long cntr, oracle_l
                 const double *p = ft + POP(c, i, j, zd - 2 - q, 0);
                 double u = 0.;
                 dens = 0.;
+                if (mask_at(c, i, j, zd - 2 - q) < 0) { /* solid inside the fluid rows */
+                    ux[q] = 0.;
+                    continue;
+                }
                 for (int a = 0; a < 19; a++) {
                     dens += p[a];
                     u += DEX[a] * p[a];

@@ -120,3 +120,28 @@ def stripes_mask(X, Y, period, slip_width, offset=0, along="x"):
     j = (np.arange(Y) - offset) % period
     row = np.where(j < slip_width, -2, -1).astype(np.int8)
     return np.ascontiguousarray(np.broadcast_to(row[None, :], (X, Y)))
+
+
+def riblet_bands(X, Y, spacing, height, offset=0.0):
+    """Solid rows of streamwise-invariant triangular riblets (spanwise
+    period `spacing`, tip height `height` cells): cell (y, z) is solid when
+    its centre z + 1/2 lies below the triangle height at y + 1/2.  Same rule
+    as dns-solvers_amd/driver/dnscase.c (dns_case_bands)."""
+    yc = np.arange(Y) + 0.5 - offset
+    phase = np.mod(yc, spacing) / spacing
+    H = height * (1.0 - np.abs(2.0 * phase - 1.0))
+    hb = np.where(H > 0.5, np.ceil(H - 0.5), 0).astype(np.int16)
+    return np.ascontiguousarray(np.broadcast_to(hb[None, :], (X, Y)))
+
+
+def band_mask(X, Y, Z, hb, ht, bottom=None, top=None):
+    """Reference node codes [X][Y][Z+2] with solid bands as -1 nodes."""
+    s = np.zeros((X, Y, Z + 2), dtype=np.int8)
+    s[:, :, 0] = -1 if bottom is None else bottom
+    s[:, :, Z + 1] = -1 if top is None else top
+    k = np.arange(Z + 2)[None, None, :]
+    hb = np.asarray(hb)[:, :, None]
+    ht = np.asarray(ht)[:, :, None]
+    s[(k >= 1) & (k <= hb)] = -1
+    s[(k <= Z) & (k >= Z + 1 - ht)] = -1
+    return s

@@ -25,7 +25,8 @@ class RunCase(ctypes.Structure):
                 ("xdv", ctypes.c_int), ("ydv", ctypes.c_int), ("zdv", ctypes.c_int),
                 ("TSTR", ctypes.c_int), ("TEND", ctypes.c_int), ("ITER", ctypes.c_long),
                 ("forcing", ctypes.c_int), ("force", ctypes.c_double), ("precision", ctypes.c_int),
-                ("omega", ctypes.c_double), ("texture", ctypes.c_int), ("walls", ctypes.c_int),
+                ("omega", ctypes.c_double), ("texture", ctypes.c_int), ("riblet_spacing", ctypes.c_double),
+                ("riblet_height", ctypes.c_double), ("riblet_offset", ctypes.c_double), ("walls", ctypes.c_int),
                 ("period_x", ctypes.c_int), ("period_y", ctypes.c_int), ("slip_x", ctypes.c_int),
                 ("slip_y", ctypes.c_int), ("offset_x", ctypes.c_int), ("offset_y", ctypes.c_int),
                 ("init", ctypes.c_int), ("init_uc", ctypes.c_double), ("init_amp", ctypes.c_double),
@@ -39,6 +40,7 @@ def lib():
     L.dns_case_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(RunCase), ctypes.c_char_p, ctypes.c_int]
     L.dns_case_defaults.argtypes = [ctypes.POINTER(RunCase)]
     L.dns_case_masks.argtypes = [ctypes.POINTER(RunCase), ctypes.c_void_p, ctypes.c_void_p]
+    L.dns_case_bands.argtypes = [ctypes.POINTER(RunCase), ctypes.c_void_p, ctypes.c_void_p]
     L.dnsio_write_turbfld.argtypes = [ctypes.c_char_p, ctypes.c_long, ctypes.c_void_p, ctypes.c_int]
     L.dnsio_write_twoD.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_int, ctypes.c_double]
     return L
@@ -114,3 +116,19 @@ def test_case_file_and_reference_textures(lib, tmp_path):
     lib.dns_case_defaults(ctypes.byref(c))
     assert lib.dns_case_read(str(bad).encode(), ctypes.byref(c), err, 512) == -1
     assert b"nonsense" in err.value
+
+
+@pytest.mark.parametrize("spacing,height,offset", [(16, 8, 0.0), (8, 4, 0.0), (12, 5, 3.0)])
+def test_riblet_voxeliser_matches_test_replica(lib, spacing, height, offset):
+    c = RunCase()
+    lib.dns_case_defaults(ctypes.byref(c))
+    c.xdv, c.ydv, c.zdv = 4, 48, 40
+    c.texture = 3
+    c.riblet_spacing, c.riblet_height, c.riblet_offset = spacing, height, offset
+    b = np.zeros((4, 48), np.int16)
+    t = np.zeros((4, 48), np.int16)
+    lib.dns_case_bands(ctypes.byref(c), b.ctypes.data, t.ctypes.data)
+    ref = fields.riblet_bands(4, 48, spacing, height, offset)
+    assert np.array_equal(b, ref) and np.array_equal(t, ref)
+    # cell centres sample the triangle: the tallest column is within one cell of the tip
+    assert height - 1 <= b.max() <= height

@@ -164,3 +164,67 @@ def test_fp32_shifted_against_fp64():
     f32 = ch32.save_aos()[0]
     assert np.abs(f32 - f64).max() <= 1e-5
     assert np.abs(l32["stress"] - l64["stress"]).max() <= 1e-3 * np.abs(l64["stress"]).max()
+
+
+def make_riblet_pair(X, Y, Z, forcing, spacing, height, F=0.0, precision=dnsgpu.FP64, Rebs=60.0, ub=0.05):
+    ch = dnsgpu.Channel(X, Y, Z, Rebs=Rebs, ub=ub, CFL=0.25, TSTR=5, forcing=forcing, force=F,
+                        precision=precision)
+    hb = fields.riblet_bands(X, Y, spacing, height)
+    ch.set_solid_bands(hb, hb)
+    s = fields.band_mask(X, Y, Z, hb, hb)
+    o = pyoracle.Oracle(X, Y, Z, pyoracle.omega(Rebs, ub, Z), ub, 0.25, TM=5,
+                        forcing=1 if forcing == dnsgpu.CONST_F else 0, F=F, mask=s)
+    return ch, o, s
+
+
+@pytest.mark.parametrize("shape,spacing,height", [((8, 16, 24), 8, 4), ((4, 24, 140), 12, 6)])
+def test_riblets_const_force_fp64_bitwise(shape, spacing, height):
+    X, Y, Z = shape
+    print(f"case: riblets const-F {shape} s={spacing} h={height}", flush=True)
+    ch, o, s = make_riblet_pair(X, Y, Z, dnsgpu.CONST_F, spacing, height, F=2e-5)
+    f0 = seeded(ch)
+    f0[s != 0] = 0.0
+    n = 40
+    logs = ch.step(n)
+    ft, f = ch.save_aos()
+    oft, of = f0.copy(), f0.copy()
+    ologs = o.run(oft, of, n)
+    assert np.array_equal(ft, oft), rel(ft, oft)
+    for k in ("K", "flowrate", "massrate", "stup", "stdown", "stress", "rho"):
+        assert np.allclose(logs[k], ologs[k], rtol=1e-10, atol=1e-300), k
+
+
+def test_riblets_const_q_fp64_and_outputs():
+    X, Y, Z = 8, 16, 24
+    print("case: riblets const-Q", flush=True)
+    ch, o, s = make_riblet_pair(X, Y, Z, dnsgpu.CONST_Q, 8, 4)
+    f0 = seeded(ch)
+    f0[s != 0] = 0.0
+    n = 150
+    logs = ch.step(n)
+    ft, _ = ch.save_aos()
+    oft, of = f0.copy(), f0.copy()
+    ologs = o.run(oft, of, n)
+    assert rel(ft, oft) <= 1e-12
+    for k in ("K", "flowrate", "massrate", "stress", "pgrad", "force", "rho"):
+        assert np.abs(logs[k] - ologs[k]).max() <= 1e-10 * np.abs(ologs[k]).max(), k
+    st = ch.statistics()
+    assert np.allclose(st, o.statistics(ft), rtol=1e-10, atol=1e-15)
+    vel, den = ch.fields()
+    ovel, oden = o.moments(ft)
+    assert np.array_equal(vel, ovel) and np.array_equal(den, oden)
+
+
+def test_riblets_fp32_against_fp64():
+    X, Y, Z = 8, 16, 32
+    print("case: riblets fp32", flush=True)
+    a, _, s = make_riblet_pair(X, Y, Z, dnsgpu.CONST_Q, 8, 4)
+    b, _, _ = make_riblet_pair(X, Y, Z, dnsgpu.CONST_Q, 8, 4, precision=dnsgpu.FP32)
+    seeded(a)
+    seeded(b)
+    la = a.step(300)
+    lb = b.step(300)
+    fa = a.save_aos()[0]
+    fb = b.save_aos()[0]
+    assert np.abs(fa - fb).max() <= 1e-5
+    assert np.abs(la["force"] - lb["force"]).max() <= 1e-3 * np.abs(la["force"]).max()

@@ -178,3 +178,41 @@ def test_slip_changes_the_answer():
     m = fields.posts_mask(X, Y)
     mk(X, Y, Z, F=1e-5, bot=m, top=m).run(b, fb, 5)
     assert np.abs(a - b).max() > 1e-6
+
+
+@pytest.mark.parametrize("spacing,height", [(8, 4), (6, 3), (16, 8)])
+def test_riblet_streaming_exactly_once_and_mass(spacing, height):
+    # solids inside the fluid rows: every fluid slot is written exactly once
+    # by the generalised rule (bounce-back whenever the target is solid) and
+    # no population is lost or created
+    X, Y, Z = 4, 16, 20
+    hb = fields.riblet_bands(X, Y, spacing, height)
+    s = fields.band_mask(X, Y, Z, hb, hb)
+    o = pyoracle.Oracle(X, Y, Z, 1.6, 0.05, 0.25, forcing=1, F=1e-5, mask=s)
+    cnt = o.coverage()
+    fluid = s == 0
+    assert (cnt[fluid] == 1).all()
+    assert (cnt[~fluid] == 0).all()
+    ft = fields.poiseuille_field(X, Y, Z, 0.05, noise=0.1, seed=8, solid=~fluid)
+    f = ft.copy()
+    m0 = ft.sum()
+    o.run(ft, f, 30)
+    assert abs(ft.sum() - m0) < 1e-13 * m0
+    assert (ft[~fluid] == 0).all()
+
+
+def test_riblets_increase_drag_over_flat_wall_at_fixed_flow():
+    # same bulk flow (constant Q), riblet walls need more force than flat
+    # walls once laminar flow has settled (the riblets narrow the channel)
+    X, Y, Z = 2, 16, 16
+    o_flat = pyoracle.Oracle(X, Y, Z, 1.5, 0.02, 0.25, forcing=0)
+    hb = fields.riblet_bands(X, Y, 8, 3)
+    o_rib = pyoracle.Oracle(X, Y, Z, 1.5, 0.02, 0.25, forcing=0, mask=fields.band_mask(X, Y, Z, hb, hb))
+    f0 = rest(X, Y, Z)
+    a, fa = f0.copy(), f0.copy()
+    b, fb = f0.copy(), f0.copy()
+    b[fields.band_mask(X, Y, Z, hb, hb) != 0] = 0
+    fb[:] = b
+    la = o_flat.run(a, fa, 2000)
+    lb = o_rib.run(b, fb, 2000)
+    assert lb["force"][-1] > 1.2 * la["force"][-1]
