@@ -22,6 +22,8 @@
 
 using namespace lbm;
 
+constexpr int kNStat = 5;  // per-plane sums of the statistics.c kernel
+
 static thread_local char g_err[512] = "";
 
 static int set_err(int code, const char* fmt, ...)
@@ -68,6 +70,7 @@ struct dnsgpu {
     int step_nblk;  // blocks of the step kernel
     int ncu;
     int step_nblk_max;  // capacity of the partial buffers for the step grid
+    int step_resident;  // step blocks resident at once on the device
     int block;
     hipStream_t stream;
     double omega, mu, dt;
@@ -167,7 +170,7 @@ static void enqueue_step_kernel(dnsgpu* h, bool stats)
     void* dst = h->buf[1 - h->cur];
     StepArgs A = step_args(h, src, dst, h->step_nblk);
     dim3 grid(h->step_nblk), block(kWave * h->step_w);
-    const size_t lds = stats ? step_lds_bytes(h->g.Z, h->step_w) : 0;
+    const size_t lds = step_lds_bytes(h->g.Z, h->step_w, stats);
 #define STEP_VARIANT(F, S, B) hipLaunchKernelGGL((step_kernel<T, F, S, B>), grid, block, lds, h->stream, A)
     if (h->has_bands) {
         if (h->poststream) {
@@ -310,6 +313,17 @@ extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
     h->ncu = ncu;
     const int waves_per_cu = c->precision == DNSGPU_FP64 ? 4 : 8;
     int sb = (ncu * waves_per_cu + h->step_w - 1) / h->step_w;
+    // never more blocks than can be resident at once (the grid loops)
+    int per = 0;
+    const size_t lds_stats = step_lds_bytes(g.Z, h->step_w, true);
+    hipError_t oe = c->precision == DNSGPU_FP64
+                        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, step_kernel<double, false, true, true>,
+                                                                       kWave * h->step_w, lds_stats)
+                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, step_kernel<float, false, true, true>,
+                                                                       kWave * h->step_w, lds_stats);
+    if (oe != hipSuccess) return fail(set_err(DNSGPU_EHIP, "occupancy query failed"));
+    h->step_resident = std::max(1, per) * ncu;
+    sb = std::min(sb, h->step_resident);
     const char* sbe = getenv("DNSGPU_STEP_BLOCKS");
     if (sbe) sb = atoi(sbe);
     h->step_nblk_max = std::max(1, (g.ncols + h->step_w - 1) / h->step_w);
@@ -329,9 +343,9 @@ extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
         {(void**)&h->wword, xy * 4},
         {(void**)&h->bands, xy * 4},
         {(void**)&h->nf, sizeof(double) * g.Z},
-        {(void**)&h->part, sizeof(double) * kNQ * g.Z * (size_t)std::max(h->nblk, h->step_nblk_max)},
+        {(void**)&h->part, sizeof(double) * kNStat * g.Z * (size_t)std::max(h->nblk, h->step_nblk_max)},
         {(void**)&h->gpart, sizeof(double) * kNG * (size_t)std::max(h->nblk, h->step_nblk_max)},
-        {(void**)&h->planes, sizeof(double) * (kNQ * g.Z + kNG)},
+        {(void**)&h->planes, sizeof(double) * (kNStat * g.Z + kNG)},
         {(void**)&h->shift, sizeof(double) * 3 * g.Z},
         {(void**)&h->means, sizeof(double) * 4 * g.Z},
         {(void**)&h->st, sizeof(DevState)},
@@ -375,6 +389,104 @@ extern "C" void dnsgpu_destroy(dnsgpu* h)
 }
 
 extern "C" size_t dnsgpu_device_bytes(const dnsgpu* h) { return h ? h->dev_bytes : 0; }
+
+// ---------------------------------------------------------------- autotuner
+
+namespace {
+struct TuneKey {
+    int X, Y, Z, prec, bands;
+    bool operator==(const TuneKey& o) const
+    {
+        return X == o.X && Y == o.Y && Z == o.Z && prec == o.prec && bands == o.bands;
+    }
+};
+std::vector<std::pair<TuneKey, int>> g_tune_cache;
+
+bool tune_lookup(const TuneKey& k, int* blocks)
+{
+    for (auto& e : g_tune_cache)
+        if (e.first == k) {
+            *blocks = e.second;
+            return true;
+        }
+    const char* path = getenv("DNSGPU_TUNE_CACHE");
+    if (!path) return false;
+    FILE* f = fopen(path, "r");
+    if (!f) return false;
+    TuneKey r;
+    int b;
+    bool found = false;
+    while (fscanf(f, "%d %d %d %d %d %d", &r.X, &r.Y, &r.Z, &r.prec, &r.bands, &b) == 6)
+        if (r == k) {
+            *blocks = b;
+            found = true;
+        }
+    fclose(f);
+    if (found) g_tune_cache.push_back({k, *blocks});
+    return found;
+}
+
+void tune_store(const TuneKey& k, int blocks)
+{
+    g_tune_cache.push_back({k, blocks});
+    const char* path = getenv("DNSGPU_TUNE_CACHE");
+    if (!path) return;
+    FILE* f = fopen(path, "a");
+    if (!f) return;
+    fprintf(f, "%d %d %d %d %d %d\n", k.X, k.Y, k.Z, k.prec, k.bands, blocks);
+    fclose(f);
+}
+}  // namespace
+
+extern "C" int dnsgpu_autotune(dnsgpu* h, int trial_steps, int* best_blocks, double* best_ms)
+{
+    if (!h || trial_steps < 1) return set_err(DNSGPU_EARG, "bad argument");
+    const TuneKey key{h->g.X, h->g.Y, h->g.Z, h->c.precision, h->has_bands};
+    int cached = 0;
+    if (tune_lookup(key, &cached)) {
+        h->step_nblk = std::max(1, std::min(cached, std::min(h->step_nblk_max, h->step_resident)));
+        if (best_blocks) *best_blocks = h->step_nblk;
+        if (best_ms) *best_ms = -1.0;
+        return 0;
+    }
+    const int saved_ps = h->poststream;
+    h->poststream = 0;  // time the steady-state kernel
+    const int wpcs[] = {2, 3, 4, 6, 8, 12, 16};
+    int best = h->step_nblk;
+    double best_t = 1e30;
+    int last = -1;
+    for (int wpc : wpcs) {
+        int nb = (h->ncu * wpc + h->step_w - 1) / h->step_w;
+        nb = std::max(1, std::min(nb, std::min(h->step_nblk_max, h->step_resident)));
+        if (nb == last) continue;
+        last = nb;
+        h->step_nblk = nb;
+        // one untimed launch, then trial_steps timed ones; buffers are not flipped
+        for (int r = 0; r <= trial_steps; r++) {
+            if (r == 1) HIPCHK(hipEventRecord(h->ev0, h->stream));
+            if (h->c.precision == DNSGPU_FP64)
+                enqueue_step_kernel<double>(h, true);
+            else
+                enqueue_step_kernel<float>(h, true);
+            HIPCHK(hipGetLastError());
+        }
+        HIPCHK(hipEventRecord(h->ev1, h->stream));
+        HIPCHK(hipEventSynchronize(h->ev1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+        const double t = ms / trial_steps;
+        if (t < best_t) {
+            best_t = t;
+            best = nb;
+        }
+    }
+    h->poststream = saved_ps;
+    h->step_nblk = best;
+    tune_store(key, best);
+    if (best_blocks) *best_blocks = best;
+    if (best_ms) *best_ms = best_t;
+    return 0;
+}
 
 extern "C" int dnsgpu_set_walls(dnsgpu* h, const int8_t* bottom, const int8_t* top)
 {

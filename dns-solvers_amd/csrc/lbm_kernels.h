@@ -40,8 +40,8 @@ using namespace d3q19;
 
 constexpr int kWave = 64;
 constexpr int kMaxWaves = 8;  // waves per block
-constexpr int kNQ = 5;        // per-plane accumulators: su sv sw sdrho sq
-constexpr int kNG = 3;        // global accumulators: jx, mass-1, budget momentum
+constexpr int kNQ = 4;        // per-plane accumulators: su sv sw sdrho
+constexpr int kNG = 4;        // global accumulators: jx, mass-1, budget momentum, sq (K)
 constexpr int kNAcc = kNQ + kNG;
 
 struct Geom {
@@ -288,19 +288,21 @@ __device__ __forceinline__ void gather_wave(const T* __restrict__ src, const Col
     }
 }
 
-// Number of waves per block of the step kernel for a given Z, so the
-// per-wave plane accumulators (kNQ*Z doubles each), the block's shift copy
-// (3*Z doubles) and the static reduction array (kMaxWaves*kNG*kWave
-// doubles) fit in 64 KiB of LDS.
-__host__ __device__ inline int step_waves(int Z)
+// Dynamic LDS of the step kernel: [W][kNG][kWave] global partials, then
+// with STATS [W][kNQ][Z] per-wave plane sums and [3][Z] shifts.
+__host__ __device__ inline size_t step_lds_bytes(int Z, int W, bool stats)
 {
-    int w = (8192 - kMaxWaves * kNG * kWave - 3 * Z) / (kNQ * Z);
-    if (w > 4) w = 4;
-    if (w < 1) w = 1;
-    return w;
+    return sizeof(double) * ((size_t)W * kNG * kWave + (stats ? (size_t)W * kNQ * Z + 3 * (size_t)Z : 0));
 }
 
-__host__ __device__ inline size_t step_lds_bytes(int Z, int W) { return sizeof(double) * ((size_t)W * kNQ * Z + 3 * Z); }
+// Waves per block of the step kernel for a given Z: at most 4, and the
+// statistics variant's LDS within 64 KiB.
+__host__ __device__ inline int step_waves(int Z)
+{
+    int w = 4;
+    while (w > 1 && step_lds_bytes(Z, w, true) > 65536) w--;
+    return w;
+}
 
 // The fused step: pull-stream + wall rule + BGK collision with force +
 // per-plane statistics + momentum budget of the next state.
@@ -316,8 +318,7 @@ __host__ __device__ inline size_t step_lds_bytes(int Z, int W) { return sizeof(d
 template <typename T, bool FIRST, bool STATS, bool BANDS>
 __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs A)
 {
-    extern __shared__ double lds[];  // STATS: [W][kNQ][Z] plane sums, then [3][Z] shifts
-    __shared__ double red[kMaxWaves][kNG][kWave];
+    extern __shared__ double lds[];  // see step_lds_bytes
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int W = blockDim.x / kWave;
@@ -325,15 +326,17 @@ __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs A)
     const int ctop = (Z - 1) / kWave, ltop = (Z - 1) % kWave;
     const T* __restrict__ src = (const T*)A.src;
     char* __restrict__ d8 = (char*)A.dst;
-    double* mine = lds + (size_t)wave * kNQ * Z;
-    double* shl = lds + (size_t)W * kNQ * Z;
+    double* red = lds;  // [W][kNG][kWave]
+    double* planes = lds + (size_t)W * kNG * kWave;
+    double* mine = planes + (size_t)wave * kNQ * Z;
+    double* shl = planes + (size_t)W * kNQ * Z;
 
     if constexpr (STATS) {
         for (int i = lane; i < kNQ * Z; i += kWave) mine[i] = 0.0;
         for (int i = threadIdx.x; i < 3 * Z; i += blockDim.x) shl[i] = A.shift[i];
         __syncthreads();
     }
-    double gjx = 0.0, gmass = 0.0, gmom = 0.0;
+    double gjx = 0.0, gmass = 0.0, gmom = 0.0, gsq = 0.0;
     const double G = *A.G;
 
     for (int it = blockIdx.x * W + wave; it < A.g.ncols; it += gridDim.x * W) {
@@ -363,20 +366,31 @@ __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs A)
             const uint32_t zoff_up = (uint32_t)(z == 0 ? 1 : z) * (uint32_t)sizeof(T);
             const uint32_t zoff_dn = (uint32_t)(z == Z - 1 ? Z - 2 : z) * (uint32_t)sizeof(T);
             T v[Q];
-            gather_wave<T, FIRST>(src, c, zoff, zoff_up, zoff_dn, bot_lane, top_lane, z, ww, v);
             // chunk touches a band of the neighbourhood (upstream rows z-1..z+1)
             bool slow = false, solid_own = false;
             if constexpr (BANDS) {
                 const int z0 = ch * kWave, z1 = min(z0 + kWave, Z);
                 slow = (maxhb > 0 && z0 - 1 < maxhb) || (maxht > 0 && z1 >= Z - maxht);
                 solid_own = band_solid(bw[1][1], z, Z);
-                if (slow && !FIRST) {
-                    static_for<1, Q>([&](auto ac) {
-                        constexpr int a = decltype(ac)::value;
-                        if (band_solid(bw[EX[a] + 1][EY[a] + 1], z - EZ[a], Z))
-                            v[a] = src[c.cb + (long long)OPP[a] * Z + z];
-                    });
-                }
+            }
+            if (BANDS && slow && !FIRST) {
+                // each lane selects its source before the load: halfway
+                // bounce-back from its own column when the upstream node is
+                // solid (band or no-slip wall row), the pull otherwise
+                const char* s8 = (const char*)src;
+                static_for<0, Q>([&](auto ac) {
+                    constexpr int a = decltype(ac)::value;
+                    const int zs = z - EZ[a];
+                    const bool solid_up =
+                        a != 0 && (zs < 0 || zs >= Z || band_solid(bw[EX[a] + 1][EY[a] + 1], zs, Z));
+                    const long long e = (long long)a * Z + colbase<EX[a], EY[a]>(c) - EZ[a];
+                    const uint32_t off = EZ[a] == 1 ? zoff_up : (EZ[a] == -1 ? zoff_dn : zoff);
+                    const char* pn = s8 + e * (long long)sizeof(T) + off;
+                    const char* pb = s8 + (c.cb + (long long)OPP[a] * Z) * (long long)sizeof(T) + zoff;
+                    v[a] = *(const T*)(solid_up ? pb : pn);
+                });
+            } else {
+                gather_wave<T, FIRST>(src, c, zoff, zoff_up, zoff_dn, bot_lane, top_lane, z, ww, v);
             }
 
             double mass_node, jx_raw, g11, g12, g13, g14;
@@ -395,7 +409,7 @@ __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs A)
                         mine[1 * Z + z] += dv;
                         mine[2 * Z + z] += dw;
                         mine[3 * Z + z] += mass_node;
-                        mine[4 * Z + z] += (du * du + dv * dv) + dw * dw;
+                        gsq += (du * du + dv * dv) + dw * dw;
                     }
                 }
 #pragma unroll
@@ -413,7 +427,7 @@ __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs A)
                         mine[1 * Z + z] += dv;
                         mine[2 * Z + z] += dw;
                         mine[3 * Z + z] += mass_node;
-                        mine[4 * Z + z] += (du * du + dv * dv) + dw * dw;
+                        gsq += (du * du + dv * dv) + dw * dw;
                     }
                 }
 #pragma unroll
@@ -458,14 +472,16 @@ __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs A)
 
     // Deterministic reductions: plane sums over the block's waves in wave
     // order, global sums over waves then a fixed butterfly over lanes.
-    red[wave][0][lane] = gjx;
-    red[wave][1][lane] = gmass;
-    red[wave][2][lane] = gmom;
+    double* myred = red + (size_t)wave * kNG * kWave;
+    myred[0 * kWave + lane] = gjx;
+    myred[1 * kWave + lane] = gmass;
+    myred[2 * kWave + lane] = gmom;
+    myred[3 * kWave + lane] = gsq;
     __syncthreads();
     if constexpr (STATS) {
         for (int i = threadIdx.x; i < kNQ * Z; i += blockDim.x) {
             double s = 0.0;
-            for (int w = 0; w < W; w++) s += lds[(size_t)w * kNQ * Z + i];
+            for (int w = 0; w < W; w++) s += planes[(size_t)w * kNQ * Z + i];
             const int q = i / Z, z = i - q * Z;
             A.part[((long long)q * Z + z) * A.nblk + blockIdx.x] = s;
         }
@@ -473,9 +489,9 @@ __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs A)
     if (wave == 0) {
 #pragma unroll
         for (int q = 0; q < kNG; q++) {
-            if (!STATS && q == 0) continue;
+            if (!STATS && (q == 0 || q == 3)) continue;
             double s = 0.0;
-            for (int w = 0; w < W; w++) s += red[w][q][lane];
+            for (int w = 0; w < W; w++) s += red[((size_t)w * kNG + q) * kWave + lane];
 #pragma unroll
             for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
             if (lane == 0) A.gpart[(long long)q * A.nblk + blockIdx.x] = s;
@@ -594,10 +610,10 @@ __global__ __launch_bounds__(kWave* kMaxWaves) void moments_kernel(StepArgs A)
             for (int a = 0; a < Q; a++) dr += (double)v[a];
             acc[3] += dr;
         }
-        acc[4] += (du * du + dv * dv) + dw * dw;
-        acc[5] += ux;
-        acc[6] += 0.0;
-        acc[7] += ux;
+        acc[kNQ + 0] += ux;  // jx
+        acc[kNQ + 1] += 0.0; // mass budget (unused for the initial state)
+        acc[kNQ + 2] += ux;  // momentum of the state itself
+        acc[kNQ + 3] += (du * du + dv * dv) + dw * dw;
     }
     block_reduce_write(A, acc, wave, lane, zc, z);
 }
@@ -633,7 +649,7 @@ struct LogRow {  // mirrors dns_steplog
 
 struct FinalizeArgs {
     const double* planes;   // [kNQ][Z] plane sums (may be null when !stats)
-    const double* globals;  // [kNG]
+    const double* globals;  // [kNG]: jx, mass-1, budget momentum, sum |u - c|^2
     double* shift;          // [3][Z] in/out
     DevState* st;
     LogRow* row;            // null: no log row
@@ -670,7 +686,7 @@ __global__ __launch_bounds__(1024) void finalize_kernel(FinalizeArgs A)
         const double mu_ = (nf * cu + s1u) / Nxy, mv = (nf * cv + s1v) / Nxy, mw = (nf * cw + s1w) / Nxy;
         const double du = mu_ - cu, dv = mv - cv, dw = mw - cw;
         uz = mu_;
-        kz = A.planes[4 * Z + t] - 2.0 * ((du * s1u + dv * s1v) + dw * s1w) + nf * ((du * du + dv * dv) + dw * dw);
+        kz = 2.0 * ((du * s1u + dv * s1v) + dw * s1w) - nf * ((du * du + dv * dv) + dw * dw);
         uav[t] = uz;
         dav[t] = (nf + A.planes[3 * Z + t]) / Nxy;
         A.shift[t] = mu_;
@@ -695,7 +711,7 @@ __global__ __launch_bounds__(1024) void finalize_kernel(FinalizeArgs A)
             for (int z = 0; z < Z; z++) srho += A.planes[3 * Z + z];
             const double denst = (A.nfluid + srho) / N;
             const double momentum = jx / N;
-            const double K = 0.5 * sK[0] / N;
+            const double K = 0.5 * (A.globals[3] - sK[0]) / N;
             const double U = sU[0] / (double)Z;
             const double stdown = (-2. * uav[0] + 3. * uav[1] - uav[2]) * (A.mu * dav[0]);
             const double stup = -(2. * uav[Z - 1] - 3. * uav[Z - 2] + uav[Z - 3]) * (A.mu * dav[Z - 3]);

@@ -26,6 +26,7 @@ EXPORTS = (
     "dnsgpu_set_solid_bands", "dnsgpu_init_rest", "dnsgpu_init_field", "dnsgpu_load_aos",
     "dnsgpu_commit_load", "dnsgpu_save_aos", "dnsgpu_get_force", "dnsgpu_step",
     "dnsgpu_statistics", "dnsgpu_fields", "dnsgpu_time_steps", "dnsgpu_device_bytes",
+    "dnsgpu_autotune",
 )
 
 
@@ -90,6 +91,7 @@ def load():
         L.dnsgpu_statistics.argtypes = [vp, dp]
         L.dnsgpu_fields.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp]
         L.dnsgpu_time_steps.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.dnsgpu_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), dp]
         L.dnsgpu_device_bytes.argtypes = [vp]
         L.dnsgpu_device_bytes.restype = ctypes.c_size_t
         _lib = L
@@ -231,6 +233,13 @@ class Channel:
         tw = ctypes.c_double()
         _chk(load().dnsgpu_time_steps(self.h, n, 1 if stats else 0, ctypes.byref(tk), ctypes.byref(tw)))
         return tk.value, tw.value
+
+    def autotune(self, trial_steps=5):
+        """Pick the step grid by timing candidates on this handle (state kept)."""
+        b = ctypes.c_int()
+        t = ctypes.c_double()
+        _chk(load().dnsgpu_autotune(self.h, trial_steps, ctypes.byref(b), ctypes.byref(t)))
+        return b.value, t.value
 
     def device_bytes(self):
         return load().dnsgpu_device_bytes(self.h)

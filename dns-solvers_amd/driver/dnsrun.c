@@ -300,6 +300,10 @@ int main(int argc, char *argv[])
         free(ht);
     }
 
+    /* pick the step grid on this case before any state exists (the trials
+     * run on the rest state and do not advance it) */
+    API(dnsgpu_autotune(h, 3, NULL, NULL));
+
     const int TM = c.TSTR;
     if (ch == 's') {
         if (c.init == 0) {

@@ -168,6 +168,18 @@ int dnsgpu_fields(dnsgpu *h, int x0, int nx, double *vel, double *den);
  * whole sequence. */
 int dnsgpu_time_steps(dnsgpu *h, int nsteps, int with_stats, double *ms_step_kernels, double *ms_wall);
 
+/* Choose the step kernel's grid (blocks resident on the device) by timing
+ * trial_steps launches of each candidate on this handle's own state and
+ * masks.  The state is not advanced: the trials read the current buffer and
+ * write the other one without flipping them, so call it before a restart's
+ * first checkpoint is written (the second checkpoint half of a freshly
+ * loaded state lives in the other buffer).  Results are cached per
+ * (grid, precision, texture) in the process and, when the environment
+ * variable DNSGPU_TUNE_CACHE names a file, across processes.  No reference
+ * counterpart (the reference has no device to tune).  best_blocks and
+ * best_ms may be NULL. */
+int dnsgpu_autotune(dnsgpu *h, int trial_steps, int *best_blocks, double *best_ms);
+
 /* Device bytes held by the handle. */
 size_t dnsgpu_device_bytes(const dnsgpu *h);
 
