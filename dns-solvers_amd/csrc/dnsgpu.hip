@@ -23,6 +23,7 @@
 using namespace lbm;
 
 constexpr int kNStat = 5;  // per-plane sums of the statistics.c kernel
+constexpr int kSlipBlocks = 256;
 
 static thread_local char g_err[512] = "";
 
@@ -62,6 +63,10 @@ struct dnsgpu {
     double* planes;   // [kNQ*Z + kNG] reduced sums (planes then globals)
     double* shift;    // [3][Z]
     double* means;    // [4][Z] statistics pass-1 means
+    double* tprof;    // [9][Z] time accumulators of the plane profiles
+    double* slip;     // [4][kSlipBlocks] slip-velocity partial sums
+    double slip_sum[4];  // host FP64 accumulators of the slip sums
+    int slip_cols[4];    // columns behind each slip sum
     DevState* st;
     LogRow* ring;
     int ring_cap;
@@ -239,6 +244,8 @@ static int refresh_force(dnsgpu* h)
     return launch_moments<float>(h);
 }
 
+static void count_slip_columns(dnsgpu* h, const int8_t* bottom, const int8_t* top);
+
 static int update_wall_words(dnsgpu* h)
 {
     const int xy = h->g.X * h->g.Y;
@@ -348,6 +355,8 @@ extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
         {(void**)&h->planes, sizeof(double) * (kNStat * g.Z + kNG)},
         {(void**)&h->shift, sizeof(double) * 3 * g.Z},
         {(void**)&h->means, sizeof(double) * 4 * g.Z},
+        {(void**)&h->tprof, sizeof(double) * 9 * g.Z},
+        {(void**)&h->slip, sizeof(double) * 4 * kSlipBlocks},
         {(void**)&h->st, sizeof(DevState)},
         {(void**)&h->ring, sizeof(LogRow) * 1024},
     };
@@ -361,6 +370,7 @@ extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
         hipMemsetAsync(h->mtop, -1, xy, h->stream) != hipSuccess ||
         hipMemsetAsync(h->bands, 0, xy * 4, h->stream) != hipSuccess ||
         hipMemsetAsync(h->st, 0, sizeof(DevState), h->stream) != hipSuccess ||
+        hipMemsetAsync(h->tprof, 0, sizeof(double) * 9 * g.Z, h->stream) != hipSuccess ||
         hipMemsetAsync(h->shift, 0, sizeof(double) * 3 * g.Z, h->stream) != hipSuccess)
         return fail(set_err(DNSGPU_EHIP, "initial memset failed"));
     if (update_wall_words(h)) return fail(DNSGPU_EHIP);
@@ -368,6 +378,7 @@ extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
     h->cur = 0;
     h->poststream = 1;
     if (hipStreamSynchronize(h->stream) != hipSuccess) return fail(set_err(DNSGPU_EHIP, "sync failed"));
+    count_slip_columns(h, nullptr, nullptr);
     rc = dnsgpu_init_rest(h);
     if (rc) return fail(rc);
     *out = h;
@@ -378,8 +389,9 @@ extern "C" void dnsgpu_destroy(dnsgpu* h)
 {
     if (!h) return;
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    void* ptrs[] = {h->buf[0], h->buf[1], h->mbot, h->mtop, h->wword, h->bands, h->nf, h->part,
-                    h->gpart,  h->planes, h->shift, h->means, h->st, h->ring, h->stage};
+    void* ptrs[] = {h->buf[0], h->buf[1], h->mbot,  h->mtop,  h->wword, h->bands, h->nf,
+                    h->part,   h->gpart,  h->planes, h->shift, h->means, h->tprof, h->slip,
+                    h->st,     h->ring,   h->stage};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -488,6 +500,17 @@ extern "C" int dnsgpu_autotune(dnsgpu* h, int trial_steps, int* best_blocks, dou
     return 0;
 }
 
+static void count_slip_columns(dnsgpu* h, const int8_t* bottom, const int8_t* top)
+{
+    const size_t xy = (size_t)h->g.X * h->g.Y;
+    for (int q = 0; q < 4; q++) h->slip_cols[q] = 0;
+    for (size_t i = 0; i < xy; i++) {
+        const int b = bottom ? bottom[i] : -1, t = top ? top[i] : -1;
+        h->slip_cols[b == -2 ? 0 : 1]++;
+        h->slip_cols[t == -2 ? 2 : 3]++;
+    }
+}
+
 extern "C" int dnsgpu_set_walls(dnsgpu* h, const int8_t* bottom, const int8_t* top)
 {
     if (!h) return set_err(DNSGPU_EARG, "null handle");
@@ -510,6 +533,7 @@ extern "C" int dnsgpu_set_walls(dnsgpu* h, const int8_t* bottom, const int8_t* t
     int rc = update_wall_words(h);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
+    count_slip_columns(h, bottom, top);
     return 0;
 }
 
@@ -745,12 +769,15 @@ static int launch_stats_pass(dnsgpu* h, int pass)
     return 0;
 }
 
-extern "C" int dnsgpu_statistics(dnsgpu* h, double* out)
+// statistics.c's two passes over the current state: leaves the plane means
+// (UAVG, VAVG, WAVG, DAVG) in h->means and m, and the fluctuation sums of
+// pass 2 (u'u', v'v', w'w', u'w', rho'rho') in h->planes and s.
+static int run_statistics(dnsgpu* h, std::vector<double>& m, std::vector<double>& s)
 {
-    if (!h || !out) return set_err(DNSGPU_EARG, "null argument");
-    const int Z = h->g.Z, zd = Z + 2;
+    const int Z = h->g.Z;
     const double nxy = (double)(h->g.X * h->g.Y);
-    std::vector<double> s(5 * (size_t)Z), m(4 * (size_t)Z);
+    s.assign(5 * (size_t)Z, 0.0);
+    m.assign(4 * (size_t)Z, 0.0);
     int rc = h->c.precision == DNSGPU_FP64 ? launch_stats_pass<double>(h, 1) : launch_stats_pass<float>(h, 1);
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(s.data(), h->planes, sizeof(double) * 5 * Z, hipMemcpyDeviceToHost, h->stream));
@@ -763,6 +790,17 @@ extern "C" int dnsgpu_statistics(dnsgpu* h, double* out)
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(s.data(), h->planes, sizeof(double) * 5 * Z, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+extern "C" int dnsgpu_statistics(dnsgpu* h, double* out)
+{
+    if (!h || !out) return set_err(DNSGPU_EARG, "null argument");
+    const int Z = h->g.Z, zd = Z + 2;
+    const double nxy = (double)(h->g.X * h->g.Y);
+    std::vector<double> s, m;
+    int rc = run_statistics(h, m, s);
+    if (rc) return rc;
     memset(out, 0, sizeof(double) * 9 * zd);
     for (int z = 0; z < Z; z++) {
         const int k = z + 1;
@@ -775,6 +813,119 @@ extern "C" int dnsgpu_statistics(dnsgpu* h, double* out)
         out[6 * zd + k] = s[3 * (size_t)Z + z] / nxy;
         out[7 * zd + k] = m[3 * (size_t)Z + z];
         out[8 * zd + k] = sqrt(s[4 * (size_t)Z + z] / nxy);
+    }
+    return 0;
+}
+
+// ---------------------------------------------------------------- time averages
+
+extern "C" int dnsgpu_tavg_reset(dnsgpu* h)
+{
+    if (!h) return set_err(DNSGPU_EARG, "null handle");
+    HIPCHK(hipMemsetAsync(h->tprof, 0, sizeof(double) * 9 * h->g.Z, h->stream));
+    DevState st;
+    HIPCHK(hipMemcpyAsync(&st, h->st, sizeof st, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const double G = st.G, rho = st.rho_mean;
+    memset(&st, 0, sizeof st);
+    st.G = G;
+    st.rho_mean = rho;
+    HIPCHK(hipMemcpyAsync(h->st, &st, sizeof st, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int q = 0; q < 4; q++) h->slip_sum[q] = 0.0;
+    return 0;
+}
+
+template <typename T>
+static int launch_slip(dnsgpu* h)
+{
+    StatArgs A;
+    A.g = h->g;
+    A.cur = h->buf[h->cur];
+    A.mbot = h->mbot;
+    A.mtop = h->mtop;
+    A.bands = h->has_bands ? h->bands : nullptr;
+    A.local = h->poststream;
+    A.pass = 0;
+    A.means = nullptr;
+    A.part = nullptr;
+    A.nblk = 0;
+    hipLaunchKernelGGL(slip_velocity_kernel<T>, dim3(kSlipBlocks), dim3(256), 0, h->stream, A, h->slip);
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+extern "C" int dnsgpu_tavg_sample(dnsgpu* h)
+{
+    if (!h) return set_err(DNSGPU_EARG, "null handle");
+    std::vector<double> s, m;
+    int rc = run_statistics(h, m, s);
+    if (rc) return rc;
+    const int Z = h->g.Z;
+    hipLaunchKernelGGL(tavg_profile_kernel, dim3((Z + 255) / 256), dim3(256), 0, h->stream, h->means, h->planes, Z,
+                       (double)h->g.X * h->g.Y, h->tprof, h->st);
+    HIPCHK(hipGetLastError());
+    rc = h->c.precision == DNSGPU_FP64 ? launch_slip<double>(h) : launch_slip<float>(h);
+    if (rc) return rc;
+    std::vector<double> part(4 * (size_t)kSlipBlocks);
+    HIPCHK(hipMemcpyAsync(part.data(), h->slip, sizeof(double) * part.size(), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int q = 0; q < 4; q++) {
+        double t = 0.0;
+        for (int b = 0; b < kSlipBlocks; b++) t += part[(size_t)q * kSlipBlocks + b];
+        h->slip_sum[q] += t;
+    }
+    return 0;
+}
+
+extern "C" int dnsgpu_tavg_get(dnsgpu* h, dns_tavg* out, double* profiles)
+{
+    if (!h || !out) return set_err(DNSGPU_EARG, "null argument");
+    const int Z = h->g.Z, zd = Z + 2;
+    DevState st;
+    std::vector<double> tp(9 * (size_t)Z);
+    HIPCHK(hipMemcpyAsync(&st, h->st, sizeof st, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(tp.data(), h->tprof, sizeof(double) * tp.size(), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    memset(out, 0, sizeof *out);
+    out->nsteps = st.n_rows;
+    out->nsamples = st.n_prof;
+    const double nr = st.n_rows > 0 ? st.n_rows : 1.0;
+    out->pgrad = st.s_pgrad / nr;
+    out->utau = sqrt(fabs(out->pgrad));  // time_stats.c:114
+    out->stress = st.s_stress / nr;
+    out->stup = st.s_stup / nr;
+    out->stdown = st.s_stdown / nr;
+    out->Ub = st.s_U / nr;
+    out->K = st.s_K / nr;
+    out->rho = st.s_rho / nr;
+    // cfriction.c:33,64: Cf = stress / (0.5 ub^2) with the time-mean ub
+    out->Cf_w = out->Ub != 0.0 ? out->stress / (0.5 * out->Ub * out->Ub) : 0.0;
+    out->Cf_p = out->Ub != 0.0 ? out->pgrad / (0.5 * out->Ub * out->Ub) : 0.0;
+    const double nu = (1. / 3.) * (1. / h->omega - 0.5);
+    out->Re_tau = out->utau * (0.5 * Z) / nu;  // time_stats.c:117
+    const double ns = st.n_prof > 0 ? st.n_prof : 1.0;
+    const double cnt[4] = {(double)h->slip_cols[0], (double)h->slip_cols[1], (double)h->slip_cols[2],
+                           (double)h->slip_cols[3]};
+    double* u4[4] = {&out->u_slip_bottom, &out->u_noslip_bottom, &out->u_slip_top, &out->u_noslip_top};
+    for (int q = 0; q < 4; q++) *u4[q] = cnt[q] > 0 ? h->slip_sum[q] / (ns * cnt[q]) : 0.0;
+    if (profiles) {
+        // turb-fld order: UAVG VAVG WAVG UTIN VTIN WTIN REYS DAVG DTIN, time-averaged
+        memset(profiles, 0, sizeof(double) * 9 * zd);
+        for (int z = 0; z < Z; z++) {
+            const int k = z + 1;
+            const double u = tp[0 * (size_t)Z + z] / ns, v = tp[1 * (size_t)Z + z] / ns;
+            const double w = tp[2 * (size_t)Z + z] / ns, r = tp[3 * (size_t)Z + z] / ns;
+            profiles[0 * zd + k] = u;
+            profiles[1 * zd + k] = v;
+            profiles[2 * zd + k] = w;
+            profiles[3 * zd + k] = sqrt(fmax(tp[4 * (size_t)Z + z] / ns - u * u, 0.0));
+            profiles[4 * zd + k] = sqrt(fmax(tp[5 * (size_t)Z + z] / ns - v * v, 0.0));
+            profiles[5 * zd + k] = sqrt(fmax(tp[6 * (size_t)Z + z] / ns - w * w, 0.0));
+            profiles[6 * zd + k] = tp[7 * (size_t)Z + z] / ns - u * w;
+            profiles[7 * zd + k] = r;
+            profiles[8 * zd + k] = sqrt(fmax(tp[8 * (size_t)Z + z] / ns - r * r, 0.0));
+        }
     }
     return 0;
 }

@@ -239,6 +239,68 @@ __global__ __launch_bounds__(kWave* kMaxWaves) void stats_kernel(StatArgs A)
     }
 }
 
+// ------------------------------------------------------- time averages
+
+// Adds one sample of the plane statistics (means from pass 1, fluctuation
+// sums from pass 2 of stats_kernel) into FP64 time accumulators:
+//   tprof[0..3][z] += plane means of u, v, w, rho
+//   tprof[4..6][z] += plane means of u^2, v^2, w^2  (variance + mean^2)
+//   tprof[7][z]    += plane mean of u w            (covariance + product)
+//   tprof[8][z]    += plane mean of rho^2
+// so the time-averaged rms is sqrt(<u^2> - <u>^2) over planes and time.
+__global__ __launch_bounds__(256) void tavg_profile_kernel(const double* __restrict__ means,
+                                                           const double* __restrict__ fl, int Z, double nxy,
+                                                           double* __restrict__ tprof, DevState* st)
+{
+    const int z = blockIdx.x * blockDim.x + threadIdx.x;
+    if (z < Z) {
+        const double u = means[z], v = means[Z + z], w = means[2 * Z + z], r = means[3 * Z + z];
+        tprof[0 * Z + z] += u;
+        tprof[1 * Z + z] += v;
+        tprof[2 * Z + z] += w;
+        tprof[3 * Z + z] += r;
+        tprof[4 * Z + z] += fl[0 * Z + z] / nxy + u * u;
+        tprof[5 * Z + z] += fl[1 * Z + z] / nxy + v * v;
+        tprof[6 * Z + z] += fl[2 * Z + z] / nxy + w * w;
+        tprof[7 * Z + z] += fl[3 * Z + z] / nxy + u * w;
+        tprof[8 * Z + z] += fl[4 * Z + z] / nxy + r * r;
+    }
+    if (z == 0) st->n_prof += 1.0;
+}
+
+// Mean streamwise velocity of the wall-adjacent fluid rows over slip
+// columns (super-hydrophobic stripes/posts) and over no-slip columns, for
+// the slip velocity; one thread per column, fixed-order block sums.
+//   out[q][blk], q = sum u over bottom-slip, bottom-noslip, top-slip, top-noslip
+template <typename T>
+__global__ __launch_bounds__(256) void slip_velocity_kernel(StatArgs A, double* __restrict__ out)
+{
+    __shared__ double sh[4][256];
+    const int X = A.g.X, Y = A.g.Y, Z = A.g.Z;
+    double acc[4] = {0., 0., 0., 0.};
+    for (int col = blockIdx.x * blockDim.x + threadIdx.x; col < X * Y; col += gridDim.x * blockDim.x) {
+        const Column c = make_column_lane(col, X, Y, Z);
+        for (int side = 0; side < 2; side++) {
+            const int z = side ? Z - 1 : 0;
+            if (A.bands && band_solid(A.bands[col], z, Z)) continue;
+            const int8_t m = side ? A.mtop[col] : A.mbot[col];
+            double f[Q];
+            state_f<T>((const T*)A.cur, c, z, A.mbot, A.mtop, A.bands, A.local != 0, f);
+            double ux, uy, uz, dens;
+            moments_fp64(f, ux, uy, uz, dens);
+            acc[2 * side + (m == -2 ? 0 : 1)] += ux / dens;
+        }
+    }
+    for (int q = 0; q < 4; q++) sh[q][threadIdx.x] = acc[q];
+    __syncthreads();
+    for (int w = 128; w >= 1; w >>= 1) {
+        if ((int)threadIdx.x < w)
+            for (int q = 0; q < 4; q++) sh[q][threadIdx.x] += sh[q][threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) out[threadIdx.x * gridDim.x + blockIdx.x] = sh[threadIdx.x][0];
+}
+
 // ------------------------------------------------------- initial fields
 
 struct InitArgs {

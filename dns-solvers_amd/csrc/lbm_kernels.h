@@ -640,7 +640,10 @@ __global__ __launch_bounds__(256) void reduce_rows_kernel(const double* __restri
 struct DevState {
     double G;         // force of the next step
     double rho_mean;  // V.rho of the last logged step
-    double pad[6];
+    // FP64 time accumulators of the logged rows since the last reset
+    double n_rows, s_pgrad, s_stress, s_stup, s_stdown, s_U, s_K, s_rho;
+    double n_prof;  // profile samples accumulated (tavg_profile_kernel)
+    double pad[7];
 };
 
 struct LogRow {  // mirrors dns_steplog
@@ -728,6 +731,14 @@ __global__ __launch_bounds__(1024) void finalize_kernel(FinalizeArgs A)
             r.rho = denst;
             *A.row = r;
             A.st->rho_mean = denst;
+            A.st->n_rows += 1.0;
+            A.st->s_pgrad += r.pgrad;
+            A.st->s_stress += r.stress;
+            A.st->s_stup += stup;
+            A.st->s_stdown += stdown;
+            A.st->s_U += U;
+            A.st->s_K += K;
+            A.st->s_rho += denst;
         }
         // force of the next step (encalc.c:121 on the next state)
         if (A.forcing == 0) {

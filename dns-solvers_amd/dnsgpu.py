@@ -26,8 +26,26 @@ EXPORTS = (
     "dnsgpu_set_solid_bands", "dnsgpu_init_rest", "dnsgpu_init_field", "dnsgpu_load_aos",
     "dnsgpu_commit_load", "dnsgpu_save_aos", "dnsgpu_get_force", "dnsgpu_step",
     "dnsgpu_statistics", "dnsgpu_fields", "dnsgpu_time_steps", "dnsgpu_device_bytes",
-    "dnsgpu_autotune",
+    "dnsgpu_autotune", "dnsgpu_tavg_reset", "dnsgpu_tavg_sample", "dnsgpu_tavg_get",
 )
+
+TAVG_FIELDS = ("nsteps", "nsamples", "pgrad", "utau", "stress", "stup", "stdown", "Ub", "K", "rho", "Cf_w",
+               "Cf_p", "Re_tau", "u_slip_bottom", "u_noslip_bottom", "u_slip_top", "u_noslip_top")
+
+
+class DnsTavg(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in TAVG_FIELDS]
+
+
+def drag_reduction(base, textured, forcing=CONST_Q):
+    """Drag reduction in percent of a textured run against a smooth base run
+    (dicts from Channel.tavg()): (Cf0 - Cf)/Cf0 * 100 under constant flow
+    rate (TKE-budget/postprocessing/cfriction.c:46,77, from the wall stress
+    and from the pressure gradient), 1 - (Ub0/Ub)^2 under constant force."""
+    if forcing == CONST_Q:
+        return {"DR_w": (base["Cf_w"] - textured["Cf_w"]) / base["Cf_w"] * 100.0,
+                "DR_p": (base["Cf_p"] - textured["Cf_p"]) / base["Cf_p"] * 100.0}
+    return {"DR_Ub": (1.0 - (base["Ub"] / textured["Ub"]) ** 2) * 100.0}
 
 
 class DnsCase(ctypes.Structure):
@@ -92,6 +110,9 @@ def load():
         L.dnsgpu_fields.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp]
         L.dnsgpu_time_steps.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp]
         L.dnsgpu_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), dp]
+        L.dnsgpu_tavg_reset.argtypes = [vp]
+        L.dnsgpu_tavg_sample.argtypes = [vp]
+        L.dnsgpu_tavg_get.argtypes = [vp, ctypes.POINTER(DnsTavg), dp]
         L.dnsgpu_device_bytes.argtypes = [vp]
         L.dnsgpu_device_bytes.restype = ctypes.c_size_t
         _lib = L
@@ -240,6 +261,21 @@ class Channel:
         t = ctypes.c_double()
         _chk(load().dnsgpu_autotune(self.h, trial_steps, ctypes.byref(b), ctypes.byref(t)))
         return b.value, t.value
+
+    def tavg_reset(self):
+        _chk(load().dnsgpu_tavg_reset(self.h))
+
+    def tavg_sample(self):
+        _chk(load().dnsgpu_tavg_sample(self.h))
+
+    def tavg(self, profiles=False):
+        t = DnsTavg()
+        prof = np.zeros((9, self.Z + 2)) if profiles else None
+        _chk(load().dnsgpu_tavg_get(self.h, ctypes.byref(t), _dp(prof) if profiles else None))
+        out = {k: getattr(t, k) for k in TAVG_FIELDS}
+        if profiles:
+            out["profiles"] = prof
+        return out
 
     def device_bytes(self):
         return load().dnsgpu_device_bytes(self.h)

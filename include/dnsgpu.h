@@ -162,6 +162,36 @@ int dnsgpu_statistics(dnsgpu *h, double *out);
  * vel[nx][ydv][zdv+2][3], den[nx][ydv][zdv+2] (zeros on wall rows). */
 int dnsgpu_fields(dnsgpu *h, int x0, int nx, double *vel, double *den);
 
+/* Time averages, FP64 accumulators on the device.  Every logged step
+ * (dnsgpu_step with rows) adds its row; dnsgpu_tavg_sample adds one sample
+ * of the statistics.c plane profiles of the current state and of the
+ * wall-adjacent velocity over slip / no-slip wall columns.  The summary
+ * follows the reference's post-processing: u_tau = sqrt(<p-grad>)
+ * (TKE-budget/postprocessing/time_stats.c:101-114), Re_tau = u_tau h / nu
+ * (time_stats.c:115-117), Cf = <tau_w>/(0.5 <Ub>^2) from w-str.txt and from
+ * p-grad.txt (cfriction.c:15-86).  profiles (may be NULL) receives
+ * [9][zdv+2] time-averaged profiles in the turb-fld order, the rms and
+ * <u'w'> taken over planes and time. */
+typedef struct dns_tavg {
+    double nsteps;   /* logged steps accumulated */
+    double nsamples; /* profile samples accumulated */
+    double pgrad;    /* <F h> */
+    double utau;     /* sqrt(<F h>) */
+    double stress;   /* <tau_w> */
+    double stup, stdown;
+    double Ub;       /* <bulk velocity> (flowrate.txt / (ydv zdv)) */
+    double K;        /* <K> */
+    double rho;      /* <V.rho> */
+    double Cf_w;     /* <tau_w> / (0.5 <Ub>^2) */
+    double Cf_p;     /* <F h> / (0.5 <Ub>^2) */
+    double Re_tau;   /* u_tau (zdv/2) / nu */
+    double u_slip_bottom, u_noslip_bottom, u_slip_top, u_noslip_top; /* <u> of the wall-adjacent rows */
+} dns_tavg;
+
+int dnsgpu_tavg_reset(dnsgpu *h);
+int dnsgpu_tavg_sample(dnsgpu *h);
+int dnsgpu_tavg_get(dnsgpu *h, dns_tavg *out, double *profiles);
+
 /* Kernel-only timing helpers for benchmarks: time nsteps steps on the
  * handle's stream with HIP events; ms_total receives the elapsed time of
  * the step kernels alone (sum over steps) and ms_wall the wall time of the

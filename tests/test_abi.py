@@ -29,3 +29,4 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(dnsgpu.DnsCase) == 72
     assert ctypes.sizeof(dnsgpu.DnsStepLog) == 80
     assert ctypes.sizeof(dnsgpu.DnsInit) == 40
+    assert ctypes.sizeof(dnsgpu.DnsTavg) == 17 * 8
