@@ -35,6 +35,8 @@ void dns_case_defaults(dns_run_case *c)
     c->init_modes = 4;
     c->riblet_spacing = 16;
     c->riblet_height = 8;
+    c->tavg_start = -1;
+    c->tavg_every = 0;
     strcpy(c->outdir, ".");
 }
 
@@ -61,9 +63,13 @@ static int set_key(dns_run_case *c, const char *k, const char *v, char *err, int
     DBL(Rebs) DBL(ub) DBL(CFL) DBL(force) DBL(omega) DBL(init_uc) DBL(init_amp)
     DBL(riblet_spacing) DBL(riblet_height) DBL(riblet_offset)
     INT(xdv) INT(ydv) INT(zdv) INT(TSTR) INT(TEND) INT(period_x) INT(period_y) INT(slip_x) INT(slip_y)
-    INT(offset_x) INT(offset_y) INT(init_modes)
+    INT(offset_x) INT(offset_y) INT(init_modes) INT(tavg_every)
 #undef DBL
 #undef INT
+    if (!strcmp(k, "tavg_start")) {
+        c->tavg_start = strtol(v, NULL, 10);
+        return 0;
+    }
     if (!strcmp(k, "ITER")) {
         c->ITER = strtol(v, NULL, 10);
         return 0;

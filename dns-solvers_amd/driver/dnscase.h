@@ -16,6 +16,8 @@
  *   walls      = both | bottom | top  (which walls carry the texture)
  *   init       = rest | poiseuille | turbulent   (for -s; rest is init.c)
  *   init_uc, init_amp, init_seed, init_modes
+ *   tavg_start = <step>               start of the time averages (default: off)
+ *   tavg_every = <steps>              profile sampling cadence (default utime)
  *   outdir     = .                    (where every output file goes)
  */
 #ifndef DNSCASE_H
@@ -42,6 +44,8 @@ typedef struct dns_run_case {
     double init_uc, init_amp;
     uint64_t init_seed;
     int init_modes;
+    long tavg_start; /* < 0: no time averages */
+    int tavg_every;
     char outdir[512];
 } dns_run_case;
 

@@ -147,3 +147,25 @@ int dnsio_read_force(const char *dir, long tlabel, double *force)
     fclose(f);
     return r == 1 ? 0 : -1;
 }
+
+int dnsio_write_tavg(const char *dir, long ts, const dns_tavg *t, const double *profiles, int zd)
+{
+    char name[64];
+    snprintf(name, sizeof name, "tavg-fld.%.3d.%.3ld", 0, ts);
+    FILE *f = open_in(dir, name, "wb");
+    if (!f) return -1;
+    size_t w = fwrite(profiles, sizeof(double), 9 * (size_t)zd, f);
+    fclose(f);
+    if (w != 9 * (size_t)zd) return -1;
+    f = open_in(dir, "tavg-summary.txt", "w");
+    if (!f) return -1;
+    fprintf(f, "steps %.0f\nprofile_samples %.0f\n", t->nsteps, t->nsamples);
+    fprintf(f, "pgrad %.12e\nu_tau %.12e\nRe_tau %.6f\n", t->pgrad, t->utau, t->Re_tau);
+    fprintf(f, "stress %.12e\nstress_top %.12e\nstress_bottom %.12e\n", t->stress, t->stup, t->stdown);
+    fprintf(f, "Ub %.12e\nK %.12e\nrho %.12e\n", t->Ub, t->K, t->rho);
+    fprintf(f, "Cf_w %.12e\nCf_p %.12e\n", t->Cf_w, t->Cf_p);
+    fprintf(f, "u_slip_bottom %.12e\nu_noslip_bottom %.12e\nu_slip_top %.12e\nu_noslip_top %.12e\n",
+            t->u_slip_bottom, t->u_noslip_bottom, t->u_slip_top, t->u_noslip_top);
+    fclose(f);
+    return 0;
+}

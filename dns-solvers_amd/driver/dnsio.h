@@ -38,6 +38,11 @@ int dnsio_write_turbfld(const char *dir, long ts, const double *stats, int zd);
 /* output_twoD.c:34-46 from the wall-normal column ux (zd values, walls 0). */
 int dnsio_write_twoD(const char *dir, const double *lb_ux, int zd, double ub);
 
+/* Time averages: tavg-fld.000.NNN (9 x zd doubles, the turb-fld layout of
+ * the time-averaged profiles) and tavg-summary.txt (u_tau, Re_tau, Cf,
+ * bulk velocity, slip velocities, sample counts). */
+int dnsio_write_tavg(const char *dir, long ts, const dns_tavg *t, const double *profiles, int zd);
+
 /* force.NNNN: one native double. */
 int dnsio_write_force(const char *dir, long tlabel, double force);
 int dnsio_read_force(const char *dir, long tlabel, double *force);

@@ -332,6 +332,7 @@ int main(int argc, char *argv[])
 
     const double t4 = now_s();
     int printed_rho = 0;
+    const int te = c.tavg_every > 0 ? c.tavg_every : ut;
     for (long ss = 0; ss <= up;) {
         if (!(ss % ut)) {
             const long tl = TM + ss / ut;
@@ -339,7 +340,16 @@ int main(int argc, char *argv[])
             write_fields(h, &c, (int)tl);
             write_statistics(h, &c, tl);
         }
+        if (c.tavg_start >= 0 && ss >= c.tavg_start) {
+            if (ss == c.tavg_start) API(dnsgpu_tavg_reset(h));
+            if (!((ss - c.tavg_start) % te)) API(dnsgpu_tavg_sample(h));
+        }
         long next = (ss / ut + 1) * (long)ut;
+        if (c.tavg_start >= 0) {
+            /* stop at the next time-average event too */
+            long ev = ss < c.tavg_start ? c.tavg_start : c.tavg_start + ((ss - c.tavg_start) / te + 1) * te;
+            if (ev < next) next = ev;
+        }
         if (next > up + 1) next = up + 1;
         const int n = (int)(next - ss);
         API(dnsgpu_step(h, ss, n, rows));
@@ -355,6 +365,19 @@ int main(int argc, char *argv[])
     }
     const double t2 = now_s();
     dnsio_logs_close(&L);
+    if (c.tavg_start >= 0 && c.tavg_start <= up) {
+        dns_tavg tv;
+        double *prof = malloc(9 * (size_t)zd * sizeof(double));
+        if (!prof) return 1;
+        API(dnsgpu_tavg_get(h, &tv, prof));
+        if (dnsio_write_tavg(c.outdir, TM + (up + 1) / ut, &tv, prof, zd)) {
+            fprintf(stderr, "dnsrun: cannot write the time averages\n");
+            return 1;
+        }
+        printf("time averages: u_tau=%g Re_tau=%g Cf_w=%g Cf_p=%g Ub=%g over %.0f steps\n", tv.utau, tv.Re_tau, tv.Cf_w,
+               tv.Cf_p, tv.Ub, tv.nsteps);
+        free(prof);
+    }
     write_twoD(h, &c);
     write_report(&c, omega, t2 - t4, utime);
     free(rows);

@@ -30,7 +30,8 @@ class RunCase(ctypes.Structure):
                 ("period_x", ctypes.c_int), ("period_y", ctypes.c_int), ("slip_x", ctypes.c_int),
                 ("slip_y", ctypes.c_int), ("offset_x", ctypes.c_int), ("offset_y", ctypes.c_int),
                 ("init", ctypes.c_int), ("init_uc", ctypes.c_double), ("init_amp", ctypes.c_double),
-                ("init_seed", ctypes.c_uint64), ("init_modes", ctypes.c_int), ("outdir", ctypes.c_char * 512)]
+                ("init_seed", ctypes.c_uint64), ("init_modes", ctypes.c_int), ("tavg_start", ctypes.c_long),
+                ("tavg_every", ctypes.c_int), ("outdir", ctypes.c_char * 512)]
 
 
 @pytest.fixture(scope="module")

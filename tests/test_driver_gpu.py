@@ -87,12 +87,32 @@ def test_driver_restart_matches_reference_outputs(tmp_path, name):
 
 def test_driver_rest_start_and_constant_force(tmp_path):
     p = dict(Rebs=60.0, ub=0.05, X=16, Y=8, Z=10, TSTR=0, CFL=0.25, N=40, slip=False)
-    write_case(tmp_path / "dns.case", p, "forcing = constf\nforce = 1e-5\nRSLIP = 1\n")
+    write_case(tmp_path / "dns.case", p, "forcing = constf\nforce = 1e-5\nRSLIP = 1\ntavg_start = 10\ntavg_every = 5\n")
     r = run(["-s"], tmp_path)
     assert r.returncode == 0, r.stderr
     pg = np.loadtxt(tmp_path / "p-grad.txt")
     assert np.allclose(pg[:, 1], 1e-5 * 10 / 2)
     assert "slip option: RSLIP" in open(tmp_path / "report.code").read()
+    # time averages over steps 10..40 (31 logged steps, 7 profile samples)
+    summ = dict(l.split() for l in open(tmp_path / "tavg-summary.txt"))
+    assert float(summ["steps"]) == 31 and float(summ["profile_samples"]) == 7
+    assert abs(float(summ["u_tau"]) - np.sqrt(1e-5 * 10 / 2)) < 1e-12
+    ws = np.loadtxt(tmp_path / "w-str.txt")[10:, 1]
+    assert abs(float(summ["stress"]) - ws.mean()) < 1e-10
+    prof = list(tmp_path.glob("tavg-fld.000.*"))
+    assert len(prof) == 1 and prof[0].stat().st_size == 9 * 12 * 8
+
+
+def test_driver_riblets(tmp_path):
+    p = dict(Rebs=60.0, ub=0.05, X=8, Y=32, Z=24, TSTR=0, CFL=0.25, N=48, slip=False)
+    write_case(tmp_path / "dns.case", p, "texture = riblets\nriblet_spacing = 8\nriblet_height = 4\n"
+                                         "init = poiseuille\ninit_uc = 0.06\ninit_amp = 0.05\n")
+    r = run(["-s"], tmp_path)
+    assert r.returncode == 0, r.stderr
+    assert "riblets" in open(tmp_path / "report.code").read()
+    den = np.fromfile(tmp_path / "den.0000", dtype=np.float64).reshape(8, 32, 26)
+    # solid riblet nodes are written as zero density, fluid ones near 1
+    assert (den[:, 4, 1:4] == 0).all() and abs(den[:, 0, 1] - 1).max() < 0.1
 
 
 def test_driver_rejects_truncated_checkpoint(tmp_path):
