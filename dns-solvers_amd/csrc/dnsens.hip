@@ -1,0 +1,587 @@
+// dnsens.hip -- host side of the ensemble entry points (include/dnsgpu.h).
+//
+// An ensemble is a set of case handles (members) that advance together on
+// one stream.  Per step there are three launches for all members:
+//     ens_step_kernel -> ens_reduce_kernel -> ens_finalize_kernel
+// driven by device tables (members, tiles, finisher arguments).  A chunk of
+// steps is captured once into a HIP graph and replayed while the member
+// set is unchanged; adding or removing a member rebuilds the tables and the
+// graph.  A member keeps its handle: buffers, masks, log ring and time
+// accumulators are the handle's own, so every dnsgpu_* query works on it
+// between dnsens_step calls.
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <vector>
+
+#include "handle.h"
+#include "lbm_ens_kernels.h"
+
+namespace {
+
+struct Member {
+    dnsgpu* h;
+    size_t bytes;    // footprint charged: the handle plus any partials of its own
+    int ntiles;
+    double* part;    // [kNQ][Z][ntiles] (the handle's buffer when it is large enough)
+    double* gpart;   // [kNG][ntiles]
+    bool own_part;   // part/gpart allocated by the ensemble
+    dnsens_member_info info;
+};
+
+struct TuneEntry {
+    int prec, zmax, tiles_log2, blocks_per_cu;
+};
+std::vector<TuneEntry> g_ens_tune;
+
+int ilog2(int v)
+{
+    int r = 0;
+    while ((1 << (r + 1)) <= v) r++;
+    return r;
+}
+
+}  // namespace
+
+struct dnsens {
+    int precision;
+    size_t budget;
+    size_t used;
+    int flags;
+    std::vector<Member> mem;
+    hipStream_t stream;
+    int ncu;
+    // device tables and their capacities
+    EnsMember* d_mem;
+    FinalizeArgs* d_fin;
+    EnsTile* d_tiles;
+    double* d_probe;
+    int cap_mem, cap_tiles;
+    std::vector<EnsMember> h_mem;
+    std::vector<FinalizeArgs> h_fin;
+    std::vector<EnsTile> h_tiles;
+    int ntiles, nrows, zmax;
+    size_t lds;
+    int resident;       // step-kernel blocks resident at once for this lds
+    int blocks_per_cu;  // grid = blocks_per_cu * ncu (capped)
+    int grid;
+    bool tables_stale;  // host tables must be rebuilt and uploaded
+    bool layout_changed;  // the graph no longer matches the launch sequence
+    hipGraph_t graph;
+    hipGraphExec_t exec;
+    int graph_steps;
+    int last_chunk;
+    hipEvent_t ev0, ev1;
+};
+
+static int default_blocks_per_cu(int precision)
+{
+    // about 8 (FP32) or 4 (FP64) waves per CU, the single-case optimum
+    return precision == DNSGPU_FP64 ? 1 : 2;
+}
+
+extern "C" int dnsens_create(int precision, size_t hbm_budget, int flags, dnsens** out)
+{
+    if (!out) return set_err(DNSGPU_EARG, "null argument");
+    *out = nullptr;
+    if (precision != DNSGPU_FP64 && precision != DNSGPU_FP32)
+        return set_err(DNSGPU_EARG, "precision %d not built", precision);
+    dnsens* e = new (std::nothrow) dnsens();
+    if (!e) return set_err(DNSGPU_ENOMEM, "host allocation failed");
+    e->precision = precision;
+    e->flags = flags;
+    auto fail = [&](int code) {
+        dnsens_destroy(e);
+        return code;
+    };
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&e->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return fail(set_err(DNSGPU_EHIP, "device query failed"));
+    if (hbm_budget == 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) != hipSuccess) return fail(set_err(DNSGPU_EHIP, "hipMemGetInfo failed"));
+        hbm_budget = fr / 10 * 9;
+    }
+    e->budget = hbm_budget;
+    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_err(DNSGPU_EHIP, "stream creation failed"));
+    if (hipEventCreate(&e->ev0) != hipSuccess || hipEventCreate(&e->ev1) != hipSuccess)
+        return fail(set_err(DNSGPU_EHIP, "event creation failed"));
+    e->blocks_per_cu = default_blocks_per_cu(precision);
+    const char* env = getenv("DNSENS_BLOCKS_PER_CU");
+    if (env && atoi(env) > 0) e->blocks_per_cu = atoi(env);
+    e->tables_stale = true;
+    e->layout_changed = true;
+    *out = e;
+    return 0;
+}
+
+static void drop_graph(dnsens* e)
+{
+    if (e->exec) (void)hipGraphExecDestroy(e->exec);
+    if (e->graph) (void)hipGraphDestroy(e->graph);
+    e->exec = nullptr;
+    e->graph = nullptr;
+    e->graph_steps = 0;
+}
+
+static void free_member(Member& m)
+{
+    if (m.own_part) {
+        if (m.part) (void)hipFree(m.part);
+        if (m.gpart) (void)hipFree(m.gpart);
+    }
+    m.part = m.gpart = nullptr;
+}
+
+extern "C" void dnsens_destroy(dnsens* e)
+{
+    if (!e) return;
+    if (e->stream) (void)hipStreamSynchronize(e->stream);
+    drop_graph(e);
+    for (auto& m : e->mem) {
+        m.h->owner = nullptr;
+        free_member(m);
+    }
+    void* ptrs[] = {e->d_mem, e->d_fin, e->d_tiles, e->d_probe};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (e->ev0) (void)hipEventDestroy(e->ev0);
+    if (e->ev1) (void)hipEventDestroy(e->ev1);
+    if (e->stream) (void)hipStreamDestroy(e->stream);
+    delete e;
+}
+
+static int find_member(const dnsens* e, const dnsgpu* h)
+{
+    for (size_t i = 0; i < e->mem.size(); i++)
+        if (e->mem[i].h == h) return (int)i;
+    return -1;
+}
+
+static int tiles_of(const dnsgpu* h) { return (h->g.ncols + kEnsTileCols - 1) / kEnsTileCols; }
+
+// Bytes a member adds on top of its handle (tile partials that do not fit
+// the handle's own partial buffers).
+static size_t extra_bytes(const dnsgpu* h)
+{
+    const size_t nt = (size_t)tiles_of(h);
+    const size_t have = (size_t)std::max(h->nblk, h->step_nblk_max);
+    if (nt <= have) return 0;
+    return sizeof(double) * ((size_t)kNQ * h->g.Z * nt + (size_t)kNG * nt);
+}
+
+extern "C" size_t dnsens_member_bytes(const dnsgpu* h)
+{
+    return h ? h->dev_bytes + extra_bytes(h) : 0;
+}
+
+extern "C" int dnsens_add(dnsens* e, dnsgpu* h)
+{
+    if (!e || !h) return set_err(DNSGPU_EARG, "null argument");
+    if (h->c.precision != e->precision)
+        return set_err(DNSGPU_EARG, "member precision %d differs from the ensemble's %d", h->c.precision,
+                       e->precision);
+    if (h->g.Z > ens_max_z()) return set_err(DNSGPU_EARG, "zdv %d above the ensemble limit %d", h->g.Z, ens_max_z());
+    if (find_member(e, h) >= 0) return set_err(DNSGPU_EARG, "handle is already a member");
+    if ((int)e->mem.size() >= DNSENS_MAX_MEMBERS)
+        return set_err(DNSGPU_EARG, "ensemble holds %d members already", DNSENS_MAX_MEMBERS);
+    const size_t need = dnsens_member_bytes(h);
+    if (e->used + need > e->budget)
+        return set_err(DNSGPU_ENOMEM, "member needs %zu bytes, %zu of %zu in use", need, e->used, e->budget);
+    // the handle's own queued work first
+    HIPCHK(hipStreamSynchronize(h->stream));
+    Member m;
+    memset(&m, 0, sizeof m);
+    m.h = h;
+    m.bytes = need;
+    m.ntiles = tiles_of(h);
+    if (extra_bytes(h)) {
+        m.own_part = true;
+        if (hipMalloc(&m.part, sizeof(double) * kNQ * h->g.Z * (size_t)m.ntiles) != hipSuccess ||
+            hipMalloc(&m.gpart, sizeof(double) * kNG * (size_t)m.ntiles) != hipSuccess) {
+            free_member(m);
+            return set_err(DNSGPU_ENOMEM, "tile partials of %d tiles", m.ntiles);
+        }
+    } else {
+        m.part = h->part;
+        m.gpart = h->gpart;
+    }
+    // a state fresh from a load or an init holds post-stream populations:
+    // its first step (local gather) runs through the single-case kernels
+    if (h->poststream) {
+        DevState st;
+        HIPCHK(hipMemcpyAsync(&st, h->st, sizeof st, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        int rc = do_step(h, true, h->ring, (long long)st.steps);
+        if (rc) {
+            free_member(m);
+            return rc;
+        }
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
+    m.info.finite = 1;
+    h->owner = e;
+    e->mem.push_back(m);
+    e->used += need;
+    e->tables_stale = true;
+    e->layout_changed = true;
+    return 0;
+}
+
+extern "C" int dnsens_remove(dnsens* e, dnsgpu* h)
+{
+    if (!e || !h) return set_err(DNSGPU_EARG, "null argument");
+    const int i = find_member(e, h);
+    if (i < 0) return set_err(DNSGPU_EARG, "handle is not a member");
+    HIPCHK(hipStreamSynchronize(e->stream));
+    e->used -= e->mem[i].bytes;
+    e->mem[i].h->owner = nullptr;
+    free_member(e->mem[i]);
+    e->mem.erase(e->mem.begin() + i);
+    e->tables_stale = true;
+    e->layout_changed = true;
+    return 0;
+}
+
+extern "C" int dnsens_count(const dnsens* e) { return e ? (int)e->mem.size() : 0; }
+extern "C" size_t dnsens_bytes_used(const dnsens* e) { return e ? e->used : 0; }
+extern "C" size_t dnsens_budget(const dnsens* e) { return e ? e->budget : 0; }
+
+template <typename T>
+static int step_occupancy(size_t lds, int* per_cu)
+{
+    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, ens_step_kernel<T>, kWave * kEnsWaves, lds));
+    return 0;
+}
+
+static int pick_grid(dnsens* e)
+{
+    e->grid = std::max(1, std::min(e->blocks_per_cu * e->ncu, std::min(e->resident, e->ntiles)));
+    return 0;
+}
+
+// Host tables from the member list (each member's current buffer is src),
+// uploaded on the ensemble stream; device capacity grows as needed.
+static int upload_tables(dnsens* e)
+{
+    const int nm = (int)e->mem.size();
+    e->h_mem.resize(nm);
+    e->h_fin.resize(nm);
+    e->h_tiles.clear();
+    int row = 0, zmax = 0;
+    for (int i = 0; i < nm; i++) {
+        Member& m = e->mem[i];
+        dnsgpu* h = m.h;
+        EnsMember& d = e->h_mem[i];
+        d.a = step_args(h, h->buf[h->cur], h->buf[1 - h->cur], m.ntiles);
+        d.a.part = m.part;
+        d.a.gpart = m.gpart;
+        d.planes = h->planes;
+        d.row0 = row;
+        d.nrows = kNQ * h->g.Z + kNG;
+        row += d.nrows;
+        FinalizeArgs F = fin_args(h, true, h->ring, 0, 0);
+        F.cntr_dev = 1;
+        e->h_fin[i] = F;
+        for (int t = 0; t < m.ntiles; t++) {
+            EnsTile tl;
+            tl.m = i;
+            tl.col0 = t * kEnsTileCols;
+            tl.ncol = std::min(kEnsTileCols, h->g.ncols - tl.col0);
+            tl.lt = t;
+            e->h_tiles.push_back(tl);
+        }
+        zmax = std::max(zmax, h->g.Z);
+    }
+    e->ntiles = (int)e->h_tiles.size();
+    e->nrows = row;
+    if (nm > e->cap_mem) {
+        if (e->d_mem) (void)hipFree(e->d_mem);
+        if (e->d_fin) (void)hipFree(e->d_fin);
+        if (e->d_probe) (void)hipFree(e->d_probe);
+        e->d_mem = nullptr;
+        e->d_fin = nullptr;
+        e->d_probe = nullptr;
+        e->cap_mem = 0;
+        const int cap = std::max(16, nm * 2);
+        if (hipMalloc(&e->d_mem, sizeof(EnsMember) * cap) != hipSuccess ||
+            hipMalloc(&e->d_fin, sizeof(FinalizeArgs) * cap) != hipSuccess ||
+            hipMalloc(&e->d_probe, sizeof(double) * 5 * cap) != hipSuccess)
+            return set_err(DNSGPU_ENOMEM, "ensemble tables");
+        e->cap_mem = cap;
+        e->layout_changed = true;
+    }
+    if (e->ntiles > e->cap_tiles) {
+        if (e->d_tiles) (void)hipFree(e->d_tiles);
+        e->d_tiles = nullptr;
+        e->cap_tiles = 0;
+        const int cap = std::max(4096, e->ntiles + e->ntiles / 2);
+        if (hipMalloc(&e->d_tiles, sizeof(EnsTile) * cap) != hipSuccess)
+            return set_err(DNSGPU_ENOMEM, "tile table of %d tiles", cap);
+        e->cap_tiles = cap;
+        e->layout_changed = true;
+    }
+    if (nm) {
+        HIPCHK(hipMemcpyAsync(e->d_mem, e->h_mem.data(), sizeof(EnsMember) * nm, hipMemcpyHostToDevice, e->stream));
+        HIPCHK(hipMemcpyAsync(e->d_fin, e->h_fin.data(), sizeof(FinalizeArgs) * nm, hipMemcpyHostToDevice,
+                              e->stream));
+        HIPCHK(hipMemcpyAsync(e->d_tiles, e->h_tiles.data(), sizeof(EnsTile) * e->ntiles, hipMemcpyHostToDevice,
+                              e->stream));
+        // the host tables are rewritten by the next upload
+        HIPCHK(hipStreamSynchronize(e->stream));
+    }
+    if (zmax != e->zmax || e->resident == 0) {
+        e->zmax = zmax;
+        e->lds = step_lds_bytes(std::max(zmax, 1), kEnsWaves, true);
+        int per = 0;
+        int rc = e->precision == DNSGPU_FP64 ? step_occupancy<double>(e->lds, &per) : step_occupancy<float>(e->lds, &per);
+        if (rc) return rc;
+        e->resident = std::max(1, per) * e->ncu;
+        e->layout_changed = true;
+    }
+    if (!getenv("DNSENS_BLOCKS_PER_CU")) {
+        e->blocks_per_cu = default_blocks_per_cu(e->precision);
+        for (auto& t : g_ens_tune)
+            if (t.prec == e->precision && t.zmax == e->zmax && t.tiles_log2 == ilog2(std::max(1, e->ntiles)))
+                e->blocks_per_cu = t.blocks_per_cu;
+    }
+    const int old_grid = e->grid;
+    pick_grid(e);
+    if (e->grid != old_grid) e->layout_changed = true;
+    e->tables_stale = false;
+    return 0;
+}
+
+template <typename T>
+static void enqueue_step_only(dnsens* e, int phase)
+{
+    hipLaunchKernelGGL(ens_step_kernel<T>, dim3(e->grid), dim3(kWave * kEnsWaves), e->lds, e->stream, e->d_mem,
+                       e->d_tiles, e->ntiles, phase);
+}
+
+// k steps of every member: step, reduction, finisher per step; log row of
+// step s in ring slot s.
+template <typename T>
+static int enqueue_steps(dnsens* e, int k)
+{
+    const int nm = (int)e->mem.size();
+    for (int s = 0; s < k; s++) {
+        enqueue_step_only<T>(e, s & 1);
+        hipLaunchKernelGGL(ens_reduce_kernel, dim3(e->nrows), dim3(256), 0, e->stream, e->d_mem, nm);
+        hipLaunchKernelGGL(ens_finalize_kernel, dim3(nm), dim3(1024), 0, e->stream, e->d_fin, s);
+    }
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+static int run_chunk(dnsens* e, int k)
+{
+    const bool use_graph = !(e->flags & DNSENS_NO_GRAPH);
+    if (!use_graph) return e->precision == DNSGPU_FP64 ? enqueue_steps<double>(e, k) : enqueue_steps<float>(e, k);
+    if (e->layout_changed || e->graph_steps != k || !e->exec) {
+        drop_graph(e);
+        HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+        int rc = e->precision == DNSGPU_FP64 ? enqueue_steps<double>(e, k) : enqueue_steps<float>(e, k);
+        hipGraph_t g = nullptr;
+        hipError_t ce = hipStreamEndCapture(e->stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (ce != hipSuccess) return set_err(DNSGPU_EHIP, "graph capture failed: %s", hipGetErrorString(ce));
+        e->graph = g;
+        HIPCHK(hipGraphInstantiate(&e->exec, e->graph, nullptr, nullptr, 0));
+        e->graph_steps = k;
+        e->layout_changed = false;
+    }
+    HIPCHK(hipGraphLaunch(e->exec, e->stream));
+    return 0;
+}
+
+static int probe(dnsens* e)
+{
+    const int nm = (int)e->mem.size();
+    if (!nm) return 0;
+    hipLaunchKernelGGL(ens_probe_kernel, dim3((nm + 63) / 64), dim3(64), 0, e->stream, e->d_fin, nm,
+                       e->last_chunk - 1, e->d_probe);
+    HIPCHK(hipGetLastError());
+    std::vector<double> p(5 * (size_t)nm);
+    HIPCHK(hipMemcpyAsync(p.data(), e->d_probe, sizeof(double) * p.size(), hipMemcpyDeviceToHost, e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    for (int i = 0; i < nm; i++) {
+        dnsens_member_info& in = e->mem[i].info;
+        in.steps = p[5 * i + 0];
+        in.n_rows = p[5 * i + 1];
+        in.mean_pgrad = in.n_rows > 0 ? p[5 * i + 2] / in.n_rows : 0.0;
+        in.force = p[5 * i + 3];
+        in.last_stress = p[5 * i + 4];
+        if (!(isfinite(in.force) && isfinite(in.last_stress) && isfinite(p[5 * i + 2]))) in.finite = 0;
+    }
+    return 0;
+}
+
+static int maybe_autotune(dnsens* e);
+
+// Enqueue nsteps steps of every member in chunks of at most kRingRows.
+static int step_chunks(dnsens* e, int nsteps)
+{
+    int done = 0;
+    while (done < nsteps) {
+        const int k = std::min(nsteps - done, kRingRows);
+        if (e->tables_stale) {
+            int rc = upload_tables(e);
+            if (rc) return rc;
+            rc = maybe_autotune(e);
+            if (rc) return rc;
+        }
+        int rc = run_chunk(e, k);
+        if (rc) return rc;
+        for (auto& m : e->mem) {
+            if (k & 1) m.h->cur = 1 - m.h->cur;
+            m.h->poststream = 0;
+        }
+        // an odd chunk swaps every member's buffers: new tables, same layout
+        if (k & 1) e->tables_stale = true;
+        e->last_chunk = k;
+        done += k;
+    }
+    return 0;
+}
+
+extern "C" int dnsens_step(dnsens* e, int nsteps)
+{
+    if (!e) return set_err(DNSGPU_EARG, "null ensemble");
+    if (nsteps < 0) return set_err(DNSGPU_EARG, "nsteps < 0");
+    if (e->mem.empty() || nsteps == 0) return 0;
+    int rc = step_chunks(e, nsteps);
+    if (rc) return rc;
+    return probe(e);
+}
+
+extern "C" int dnsens_info(dnsens* e, const dnsgpu* h, dnsens_member_info* out)
+{
+    if (!e || !h || !out) return set_err(DNSGPU_EARG, "null argument");
+    const int i = find_member(e, h);
+    if (i < 0) return set_err(DNSGPU_EARG, "handle is not a member");
+    *out = e->mem[i].info;
+    return 0;
+}
+
+extern "C" int dnsens_rows(dnsens* e, const dnsgpu* h, dns_steplog* rows, int n)
+{
+    if (!e || !h || !rows) return set_err(DNSGPU_EARG, "null argument");
+    const int i = find_member(e, h);
+    if (i < 0) return set_err(DNSGPU_EARG, "handle is not a member");
+    if (n < 0 || n > e->last_chunk)
+        return set_err(DNSGPU_EARG, "%d rows asked, the last dnsens_step chunk kept %d", n, e->last_chunk);
+    static_assert(sizeof(LogRow) == sizeof(dns_steplog), "log row layout");
+    HIPCHK(hipMemcpyAsync(rows, h->ring + (e->last_chunk - n), sizeof(LogRow) * n, hipMemcpyDeviceToHost,
+                          e->stream));
+    HIPCHK(hipStreamSynchronize(e->stream));
+    return 0;
+}
+
+// ---------------------------------------------------------------- autotuner
+
+// Time trial launches of the step kernel at each candidate grid.  The
+// trials read each member's current buffer and write the other one, which
+// the next real step overwrites: the state, the partials' next use and the
+// time accumulators are unaffected.
+template <typename T>
+static int tune_grid(dnsens* e, int trials, int* best_bpc, double* best_ms)
+{
+    const int cands[] = {1, 2, 3, 4, 6, 8};
+    double bt = 1e30;
+    int bb = e->blocks_per_cu;
+    int last = -1;
+    for (int bpc : cands) {
+        e->blocks_per_cu = bpc;
+        pick_grid(e);
+        if (e->grid == last) continue;
+        last = e->grid;
+        for (int r = 0; r <= trials; r++) {
+            if (r == 1) HIPCHK(hipEventRecord(e->ev0, e->stream));
+            enqueue_step_only<T>(e, 0);
+        }
+        HIPCHK(hipGetLastError());
+        HIPCHK(hipEventRecord(e->ev1, e->stream));
+        HIPCHK(hipEventSynchronize(e->ev1));
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, e->ev0, e->ev1));
+        if (ms / trials < bt) {
+            bt = ms / trials;
+            bb = bpc;
+        }
+    }
+    e->blocks_per_cu = bb;
+    pick_grid(e);
+    *best_bpc = bb;
+    *best_ms = bt;
+    return 0;
+}
+
+static int tune_now(dnsens* e, int trials, int* best_blocks, double* best_ms)
+{
+    if (e->tables_stale) {
+        int rc = upload_tables(e);
+        if (rc) return rc;
+    }
+    int bpc = e->blocks_per_cu;
+    double ms = -1.0;
+    if (!e->mem.empty()) {
+        int rc = e->precision == DNSGPU_FP64 ? tune_grid<double>(e, trials, &bpc, &ms)
+                                             : tune_grid<float>(e, trials, &bpc, &ms);
+        if (rc) return rc;
+        g_ens_tune.push_back({e->precision, e->zmax, ilog2(std::max(1, e->ntiles)), bpc});
+    }
+    e->layout_changed = true;
+    if (best_blocks) *best_blocks = e->grid;
+    if (best_ms) *best_ms = ms;
+    return 0;
+}
+
+static int maybe_autotune(dnsens* e)
+{
+    if (!(e->flags & DNSENS_AUTOTUNE) || getenv("DNSENS_BLOCKS_PER_CU")) return 0;
+    for (auto& t : g_ens_tune)
+        if (t.prec == e->precision && t.zmax == e->zmax && t.tiles_log2 == ilog2(std::max(1, e->ntiles))) return 0;
+    return tune_now(e, 3, nullptr, nullptr);
+}
+
+extern "C" int dnsens_autotune(dnsens* e, int trial_steps, int* best_blocks, double* best_ms)
+{
+    if (!e || trial_steps < 1) return set_err(DNSGPU_EARG, "bad argument");
+    return tune_now(e, trial_steps, best_blocks, best_ms);
+}
+
+extern "C" int dnsens_time_steps(dnsens* e, int nsteps, double* ms)
+{
+    if (!e || nsteps < 1 || !ms) return set_err(DNSGPU_EARG, "bad argument");
+    if (e->mem.empty()) return set_err(DNSGPU_EARG, "empty ensemble");
+    if (e->tables_stale) {
+        int rc = upload_tables(e);
+        if (rc) return rc;
+    }
+    HIPCHK(hipEventRecord(e->ev0, e->stream));
+    int rc = step_chunks(e, nsteps);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(e->ev1, e->stream));
+    HIPCHK(hipEventSynchronize(e->ev1));
+    float t = 0.f;
+    HIPCHK(hipEventElapsedTime(&t, e->ev0, e->ev1));
+    *ms = t;
+    return probe(e);
+}
+
+extern "C" int dnsens_grid(const dnsens* e, int* blocks, int* tiles)
+{
+    if (!e) return set_err(DNSGPU_EARG, "null ensemble");
+    if (blocks) *blocks = e->grid;
+    if (tiles) *tiles = e->ntiles;
+    return 0;
+}

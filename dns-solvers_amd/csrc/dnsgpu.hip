@@ -16,18 +16,13 @@
 #include <algorithm>
 #include <vector>
 
-#include "../../include/dnsgpu.h"
-#include "lbm_io_kernels.h"
-#include "lbm_kernels.h"
+#include "handle.h"
 
 using namespace lbm;
 
-constexpr int kNStat = 5;  // per-plane sums of the statistics.c kernel
-constexpr int kSlipBlocks = 256;
-
 static thread_local char g_err[512] = "";
 
-static int set_err(int code, const char* fmt, ...)
+int set_err(int code, const char* fmt, ...)
 {
     va_list ap;
     va_start(ap, fmt);
@@ -35,55 +30,6 @@ static int set_err(int code, const char* fmt, ...)
     va_end(ap);
     return code;
 }
-
-#define HIPCHK(expr)                                                                          \
-    do {                                                                                      \
-        hipError_t e_ = (expr);                                                               \
-        if (e_ != hipSuccess)                                                                 \
-            return set_err(DNSGPU_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
-                           __FILE__, __LINE__);                                               \
-    } while (0)
-
-struct dnsgpu {
-    dns_case c;
-    Geom g;
-    int esize;  // bytes per stored population
-    void* buf[2];
-    int cur;         // buffer holding the current state
-    int poststream;  // 1: buf[cur] holds post-stream populations (after load/init)
-    int8_t* mbot;
-    int8_t* mtop;
-    uint32_t* wword;
-    uint32_t* bands;  // [X*Y] solid band words hb | ht << 16
-    double* nf;       // [Z] fluid nodes per plane
-    double nfluid;    // fluid nodes in total
-    int has_bands;
-    double* part;     // [kNQ][Z][nblk]
-    double* gpart;    // [kNG][nblk]
-    double* planes;   // [kNQ*Z + kNG] reduced sums (planes then globals)
-    double* shift;    // [3][Z]
-    double* means;    // [4][Z] statistics pass-1 means
-    double* tprof;    // [9][Z] time accumulators of the plane profiles
-    double* slip;     // [4][kSlipBlocks] slip-velocity partial sums
-    double slip_sum[4];  // host FP64 accumulators of the slip sums
-    int slip_cols[4];    // columns behind each slip sum
-    DevState* st;
-    LogRow* ring;
-    int ring_cap;
-    int nblk;       // blocks of the chunk-mapped kernels (moments, statistics)
-    int step_w;     // waves per block of the step kernel
-    int step_nblk;  // blocks of the step kernel
-    int ncu;
-    int step_nblk_max;  // capacity of the partial buffers for the step grid
-    int step_resident;  // step blocks resident at once on the device
-    int block;
-    hipStream_t stream;
-    double omega, mu, dt;
-    double* stage;
-    size_t stage_bytes;
-    size_t dev_bytes;
-    hipEvent_t ev0, ev1;
-};
 
 extern "C" const char* dnsgpu_last_error(void) { return g_err; }
 
@@ -112,7 +58,7 @@ static int stage_reserve(dnsgpu* h, size_t bytes)
     return 0;
 }
 
-static StepArgs step_args(dnsgpu* h, const void* src, void* dst, int nblk)
+StepArgs step_args(dnsgpu* h, const void* src, void* dst, int nblk)
 {
     StepArgs A;
     A.g = h->g;
@@ -131,7 +77,7 @@ static StepArgs step_args(dnsgpu* h, const void* src, void* dst, int nblk)
     return A;
 }
 
-static FinalizeArgs fin_args(dnsgpu* h, bool stats, LogRow* row, long long cntr, int from_moments)
+FinalizeArgs fin_args(dnsgpu* h, bool stats, LogRow* row, long long cntr, int from_moments)
 {
     FinalizeArgs F;
     F.planes = stats ? h->planes : nullptr;
@@ -152,6 +98,7 @@ static FinalizeArgs fin_args(dnsgpu* h, bool stats, LogRow* row, long long cntr,
     F.from_moments = from_moments;
     F.nf = h->has_bands ? h->nf : nullptr;
     F.nfluid = h->has_bands ? h->nfluid : (double)h->g.X * h->g.Y * h->g.Z;
+    F.cntr_dev = 0;
     return F;
 }
 
@@ -212,7 +159,7 @@ static int launch_step(dnsgpu* h, bool stats, LogRow* row, long long cntr)
     return 0;
 }
 
-static int do_step(dnsgpu* h, bool stats, LogRow* row, long long cntr)
+int do_step(dnsgpu* h, bool stats, LogRow* row, long long cntr)
 {
     if (h->c.precision == DNSGPU_FP64) return launch_step<double>(h, stats, row, cntr);
     return launch_step<float>(h, stats, row, cntr);
@@ -255,10 +202,9 @@ static int update_wall_words(dnsgpu* h)
     return 0;
 }
 
-extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
+// Geometry, constants and launch grids of a case (no allocation).
+static int plan_handle(const dns_case* c, dnsgpu* h)
 {
-    if (!c || !out) return set_err(DNSGPU_EARG, "null argument");
-    *out = nullptr;
     if (c->xdv < 2 || c->ydv < 2 || c->zdv < 4 || c->zdv > kWave * kMaxWaves)
         return set_err(DNSGPU_EARG, "grid %dx%dx%d unsupported (xdv,ydv>=2, 4<=zdv<=%d)", c->xdv, c->ydv, c->zdv,
                        kWave * kMaxWaves);
@@ -266,8 +212,6 @@ extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
         return set_err(DNSGPU_EARG, "grid has more than 2^31 nodes");
     if (c->precision != DNSGPU_FP64 && c->precision != DNSGPU_FP32)
         return set_err(DNSGPU_EARG, "precision %d not built", c->precision);
-    dnsgpu* h = (dnsgpu*)calloc(1, sizeof(dnsgpu));
-    if (!h) return set_err(DNSGPU_ENOMEM, "host allocation failed");
     h->c = *c;
     Geom& g = h->g;
     g.X = c->xdv;
@@ -293,42 +237,30 @@ extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
     h->mu = (1. / 6.) * (2. / h->omega - 1);
     h->dt = 1. / ((double)(0.5 * (zd - 2.) / c->CFL));
 
-    int rc = 0;
-    auto fail = [&](int code) {
-        dnsgpu_destroy(h);
-        return code;
-    };
-    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
-        return fail(set_err(DNSGPU_EHIP, "stream creation failed"));
-    if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
-        return fail(set_err(DNSGPU_EHIP, "event creation failed"));
     // grids: the chunk-mapped kernels fill the device once; the step kernel
     // keeps about 4 (FP64) or 8 (FP32) waves per CU in flight, the measured
     // optimum of the read+write stream (overridden by the autotuner or
     // DNSGPU_STEP_BLOCKS)
     int occ = 0;
-    if (c->precision == DNSGPU_FP64)
-        rc = occupancy_blocks(moments_kernel<double, false>, h->block, &occ);
-    else
-        rc = occupancy_blocks(moments_kernel<float, false>, h->block, &occ);
-    if (rc) return fail(rc);
+    int rc = c->precision == DNSGPU_FP64 ? occupancy_blocks(moments_kernel<double, false>, h->block, &occ)
+                                         : occupancy_blocks(moments_kernel<float, false>, h->block, &occ);
+    if (rc) return rc;
     h->nblk = std::max(1, std::min(occ, g.ngroups));
     int dev = 0, ncu = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-        return fail(set_err(DNSGPU_EHIP, "device query failed"));
+    HIPCHK(hipGetDevice(&dev));
+    HIPCHK(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
     h->ncu = ncu;
     const int waves_per_cu = c->precision == DNSGPU_FP64 ? 4 : 8;
     int sb = (ncu * waves_per_cu + h->step_w - 1) / h->step_w;
     // never more blocks than can be resident at once (the grid loops)
     int per = 0;
     const size_t lds_stats = step_lds_bytes(g.Z, h->step_w, true);
-    hipError_t oe = c->precision == DNSGPU_FP64
-                        ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, step_kernel<double, false, true, true>,
-                                                                       kWave * h->step_w, lds_stats)
-                        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, step_kernel<float, false, true, true>,
-                                                                       kWave * h->step_w, lds_stats);
-    if (oe != hipSuccess) return fail(set_err(DNSGPU_EHIP, "occupancy query failed"));
+    if (c->precision == DNSGPU_FP64)
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, step_kernel<double, false, true, true>,
+                                                            kWave * h->step_w, lds_stats));
+    else
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, step_kernel<float, false, true, true>,
+                                                            kWave * h->step_w, lds_stats));
     h->step_resident = std::max(1, per) * ncu;
     sb = std::min(sb, h->step_resident);
     const char* sbe = getenv("DNSGPU_STEP_BLOCKS");
@@ -336,13 +268,22 @@ extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
     h->step_nblk_max = std::max(1, (g.ncols + h->step_w - 1) / h->step_w);
     h->step_nblk_max = std::min(h->step_nblk_max, 64 * ncu);
     h->step_nblk = std::max(1, std::min(sb, h->step_nblk_max));
+    return 0;
+}
 
+struct DevAlloc {
+    void** p;
+    size_t n;
+};
+
+// Every device allocation of a planned handle (the footprint).
+static std::vector<DevAlloc> handle_allocs(dnsgpu* h)
+{
+    const Geom& g = h->g;
     const size_t popbytes = (size_t)g.total * h->esize;
     const size_t xy = (size_t)g.X * g.Y;
-    struct {
-        void** p;
-        size_t n;
-    } allocs[] = {
+    const size_t np = (size_t)std::max(h->nblk, h->step_nblk_max);
+    return {
         {&h->buf[0], popbytes},
         {&h->buf[1], popbytes},
         {(void**)&h->mbot, xy},
@@ -350,22 +291,53 @@ extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
         {(void**)&h->wword, xy * 4},
         {(void**)&h->bands, xy * 4},
         {(void**)&h->nf, sizeof(double) * g.Z},
-        {(void**)&h->part, sizeof(double) * kNStat * g.Z * (size_t)std::max(h->nblk, h->step_nblk_max)},
-        {(void**)&h->gpart, sizeof(double) * kNG * (size_t)std::max(h->nblk, h->step_nblk_max)},
+        {(void**)&h->part, sizeof(double) * kNStat * g.Z * np},
+        {(void**)&h->gpart, sizeof(double) * kNG * np},
         {(void**)&h->planes, sizeof(double) * (kNStat * g.Z + kNG)},
         {(void**)&h->shift, sizeof(double) * 3 * g.Z},
         {(void**)&h->means, sizeof(double) * 4 * g.Z},
         {(void**)&h->tprof, sizeof(double) * 9 * g.Z},
         {(void**)&h->slip, sizeof(double) * 4 * kSlipBlocks},
         {(void**)&h->st, sizeof(DevState)},
-        {(void**)&h->ring, sizeof(LogRow) * 1024},
+        {(void**)&h->ring, sizeof(LogRow) * kRingRows},
     };
-    for (auto& a : allocs) {
+}
+
+extern "C" size_t dnsgpu_footprint(const dns_case* c)
+{
+    if (!c) return 0;
+    dnsgpu p;
+    memset(&p, 0, sizeof p);
+    if (plan_handle(c, &p)) return 0;
+    size_t n = 0;
+    for (auto& a : handle_allocs(&p)) n += a.n;
+    return n;
+}
+
+extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
+{
+    if (!c || !out) return set_err(DNSGPU_EARG, "null argument");
+    *out = nullptr;
+    dnsgpu* h = (dnsgpu*)calloc(1, sizeof(dnsgpu));
+    if (!h) return set_err(DNSGPU_ENOMEM, "host allocation failed");
+    auto fail = [&](int code) {
+        dnsgpu_destroy(h);
+        return code;
+    };
+    int rc = plan_handle(c, h);
+    if (rc) return fail(rc);
+    const Geom& g = h->g;
+    if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
+        return fail(set_err(DNSGPU_EHIP, "stream creation failed"));
+    if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
+        return fail(set_err(DNSGPU_EHIP, "event creation failed"));
+    for (auto& a : handle_allocs(h)) {
         if (hipMalloc(a.p, a.n) != hipSuccess)
             return fail(set_err(DNSGPU_ENOMEM, "device allocation of %zu bytes failed", a.n));
         h->dev_bytes += a.n;
     }
-    h->ring_cap = 1024;
+    const size_t xy = (size_t)g.X * g.Y;
+    h->ring_cap = kRingRows;
     if (hipMemsetAsync(h->mbot, -1, xy, h->stream) != hipSuccess ||
         hipMemsetAsync(h->mtop, -1, xy, h->stream) != hipSuccess ||
         hipMemsetAsync(h->bands, 0, xy * 4, h->stream) != hipSuccess ||
@@ -388,6 +360,7 @@ extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
 extern "C" void dnsgpu_destroy(dnsgpu* h)
 {
     if (!h) return;
+    if (h->owner) (void)dnsens_remove(h->owner, h);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
     void* ptrs[] = {h->buf[0], h->buf[1], h->mbot,  h->mtop,  h->wword, h->bands, h->nf,
                     h->part,   h->gpart,  h->planes, h->shift, h->means, h->tprof, h->slip,
@@ -453,6 +426,7 @@ void tune_store(const TuneKey& k, int blocks)
 extern "C" int dnsgpu_autotune(dnsgpu* h, int trial_steps, int* best_blocks, double* best_ms)
 {
     if (!h || trial_steps < 1) return set_err(DNSGPU_EARG, "bad argument");
+    NOT_MEMBER(h);
     const TuneKey key{h->g.X, h->g.Y, h->g.Z, h->c.precision, h->has_bands};
     int cached = 0;
     if (tune_lookup(key, &cached)) {
@@ -514,6 +488,7 @@ static void count_slip_columns(dnsgpu* h, const int8_t* bottom, const int8_t* to
 extern "C" int dnsgpu_set_walls(dnsgpu* h, const int8_t* bottom, const int8_t* top)
 {
     if (!h) return set_err(DNSGPU_EARG, "null handle");
+    NOT_MEMBER(h);
     const size_t xy = (size_t)h->g.X * h->g.Y;
     const int8_t* src[2] = {bottom, top};
     int8_t* dst[2] = {h->mbot, h->mtop};
@@ -540,6 +515,7 @@ extern "C" int dnsgpu_set_walls(dnsgpu* h, const int8_t* bottom, const int8_t* t
 extern "C" int dnsgpu_set_solid_bands(dnsgpu* h, const int16_t* bottom, const int16_t* top)
 {
     if (!h) return set_err(DNSGPU_EARG, "null handle");
+    NOT_MEMBER(h);
     const int X = h->g.X, Y = h->g.Y, Z = h->g.Z;
     const size_t xy = (size_t)X * Y;
     std::vector<uint32_t> bw(xy, 0u);
@@ -609,12 +585,14 @@ static int launch_init(dnsgpu* h, int kind, const dns_init* spec)
 extern "C" int dnsgpu_init_rest(dnsgpu* h)
 {
     if (!h) return set_err(DNSGPU_EARG, "null handle");
+    NOT_MEMBER(h);
     return launch_init(h, 0, nullptr);
 }
 
 extern "C" int dnsgpu_init_field(dnsgpu* h, const dns_init* spec)
 {
     if (!h || !spec) return set_err(DNSGPU_EARG, "null argument");
+    NOT_MEMBER(h);
     if (spec->kind < 0 || spec->kind > 2) return set_err(DNSGPU_EARG, "unknown init kind %d", spec->kind);
     if (spec->kind == 2 && (spec->nmodes < 1 || spec->nmodes > 31))
         return set_err(DNSGPU_EARG, "nmodes must be in [1, 31]");
@@ -631,6 +609,7 @@ static int check_slab(dnsgpu* h, int x0, int nx)
 extern "C" int dnsgpu_load_aos(dnsgpu* h, int x0, int nx, const double* ftemp, const double* f)
 {
     if (!h || !ftemp || !f) return set_err(DNSGPU_EARG, "null argument");
+    NOT_MEMBER(h);
     int rc = check_slab(h, x0, nx);
     if (rc) return rc;
     const size_t half = (size_t)nx * h->g.Y * (h->g.Z + 2) * Q * sizeof(double);
@@ -662,6 +641,7 @@ extern "C" int dnsgpu_load_aos(dnsgpu* h, int x0, int nx, const double* ftemp, c
 extern "C" int dnsgpu_commit_load(dnsgpu* h)
 {
     if (!h) return set_err(DNSGPU_EARG, "null handle");
+    NOT_MEMBER(h);
     h->cur = 0;
     h->poststream = 1;
     int rc = refresh_force(h);
@@ -725,6 +705,7 @@ static int read_rows(dnsgpu* h, dns_steplog* rows, int n)
 extern "C" int dnsgpu_step(dnsgpu* h, long cntr0, int nsteps, dns_steplog* rows)
 {
     if (!h) return set_err(DNSGPU_EARG, "null handle");
+    NOT_MEMBER(h);
     if (nsteps < 0) return set_err(DNSGPU_EARG, "nsteps < 0");
     int done = 0;
     while (done < nsteps) {
@@ -826,10 +807,11 @@ extern "C" int dnsgpu_tavg_reset(dnsgpu* h)
     DevState st;
     HIPCHK(hipMemcpyAsync(&st, h->st, sizeof st, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    const double G = st.G, rho = st.rho_mean;
+    const double G = st.G, rho = st.rho_mean, steps = st.steps;
     memset(&st, 0, sizeof st);
     st.G = G;
     st.rho_mean = rho;
+    st.steps = steps;
     HIPCHK(hipMemcpyAsync(h->st, &st, sizeof st, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     for (int q = 0; q < 4; q++) h->slip_sum[q] = 0.0;
@@ -963,6 +945,7 @@ extern "C" int dnsgpu_fields(dnsgpu* h, int x0, int nx, double* vel, double* den
 extern "C" int dnsgpu_time_steps(dnsgpu* h, int nsteps, int with_stats, double* ms_kernels, double* ms_wall)
 {
     if (!h || nsteps < 1) return set_err(DNSGPU_EARG, "bad argument");
+    NOT_MEMBER(h);
     std::vector<hipEvent_t> ev(2 * (size_t)nsteps);
     for (auto& e : ev) HIPCHK(hipEventCreate(&e));
     HIPCHK(hipEventRecord(h->ev0, h->stream));

@@ -1,0 +1,84 @@
+// handle.h -- the library's internal view of a case handle, shared by the
+// single-case entry points (dnsgpu.hip) and the ensemble (dnsens.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "../../include/dnsgpu.h"
+#include "lbm_io_kernels.h"
+#include "lbm_kernels.h"
+
+using namespace lbm;
+
+constexpr int kNStat = 5;  // per-plane sums of the statistics.c kernel
+constexpr int kSlipBlocks = 256;
+constexpr int kRingRows = 1024;  // log rows kept on the device per handle
+
+// Records the message for dnsgpu_last_error and returns code.
+__attribute__((visibility("hidden"))) int set_err(int code, const char* fmt, ...);
+
+#define HIPCHK(expr)                                                                          \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return set_err(DNSGPU_EHIP, "%s failed: %s (%s:%d)", #expr, hipGetErrorString(e_), \
+                           __FILE__, __LINE__);                                               \
+    } while (0)
+
+struct dnsgpu {
+    dns_case c;
+    Geom g;
+    int esize;  // bytes per stored population
+    void* buf[2];
+    int cur;         // buffer holding the current state
+    int poststream;  // 1: buf[cur] holds post-stream populations (after load/init)
+    int8_t* mbot;
+    int8_t* mtop;
+    uint32_t* wword;
+    uint32_t* bands;  // [X*Y] solid band words hb | ht << 16
+    double* nf;       // [Z] fluid nodes per plane
+    double nfluid;    // fluid nodes in total
+    int has_bands;
+    double* part;     // [kNQ][Z][nblk]
+    double* gpart;    // [kNG][nblk]
+    double* planes;   // [kNQ*Z + kNG] reduced sums (planes then globals)
+    double* shift;    // [3][Z]
+    double* means;    // [4][Z] statistics pass-1 means
+    double* tprof;    // [9][Z] time accumulators of the plane profiles
+    double* slip;     // [4][kSlipBlocks] slip-velocity partial sums
+    double slip_sum[4];  // host FP64 accumulators of the slip sums
+    int slip_cols[4];    // columns behind each slip sum
+    DevState* st;
+    LogRow* ring;
+    int ring_cap;
+    int nblk;       // blocks of the chunk-mapped kernels (moments, statistics)
+    int step_w;     // waves per block of the step kernel
+    int step_nblk;  // blocks of the step kernel
+    int ncu;
+    int step_nblk_max;  // capacity of the partial buffers for the step grid
+    int step_resident;  // step blocks resident at once on the device
+    int block;
+    hipStream_t stream;
+    double omega, mu, dt;
+    double* stage;
+    size_t stage_bytes;
+    size_t dev_bytes;
+    hipEvent_t ev0, ev1;
+    struct dnsens* owner;  // ensemble the handle is a member of, or null
+};
+
+// Entry points that change or advance a member's state are refused while
+// the handle belongs to an ensemble (the ensemble advances it).
+#define NOT_MEMBER(h)                                                                  \
+    do {                                                                               \
+        if ((h)->owner)                                                                \
+            return set_err(DNSGPU_EARG, "%s: the handle is an ensemble member", __func__); \
+    } while (0)
+
+// Step arguments of the handle's current state (src) into dst.
+__attribute__((visibility("hidden"))) StepArgs step_args(dnsgpu* h, const void* src, void* dst, int nblk);
+// Finisher arguments of the handle (row: log row slot or null).
+__attribute__((visibility("hidden"))) FinalizeArgs fin_args(dnsgpu* h, bool stats, LogRow* row,
+                                                                long long cntr, int from_moments);
+// One single-case step on the handle's stream (kernels enqueued, no sync).
+__attribute__((visibility("hidden"))) int do_step(dnsgpu* h, bool stats, LogRow* row, long long cntr);

@@ -55,7 +55,7 @@ struct AosArgs {
 // reference never touches them; an init or checkpoint from the reference
 // holds zeros there).
 template <typename T>
-__global__ __launch_bounds__(256) void save_aos_kernel(AosArgs A)
+static __global__ __launch_bounds__(256) void save_aos_kernel(AosArgs A)
 {
     const int Z = A.g.Z, Y = A.g.Y, X = A.g.X;
     const int Zd = Z + 2;
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(256) void save_aos_kernel(AosArgs A)
 
 // lbf halves -> device: ftemp into cur_w, f into other_w (both local).
 template <typename T>
-__global__ __launch_bounds__(256) void load_aos_kernel(AosArgs A)
+static __global__ __launch_bounds__(256) void load_aos_kernel(AosArgs A)
 {
     const int Z = A.g.Z, Y = A.g.Y;
     const long long total = (long long)A.nx * Y * Z;
@@ -131,7 +131,7 @@ struct FieldArgs {
 
 // vel_print.c:65-99: u = (sum e f)/dens, zeros where dens <= 0 (wall rows).
 template <typename T>
-__global__ __launch_bounds__(256) void fields_kernel(FieldArgs A)
+static __global__ __launch_bounds__(256) void fields_kernel(FieldArgs A)
 {
     const int Z = A.g.Z, Y = A.g.Y, X = A.g.X;
     const int Zd = Z + 2;
@@ -184,7 +184,7 @@ struct StatArgs {
 // Two-pass plane statistics (statistics.c:31-100); per-lane sums over the
 // wave's columns, then fixed-order block sums written as partials.
 template <typename T>
-__global__ __launch_bounds__(kWave* kMaxWaves) void stats_kernel(StatArgs A)
+static __global__ __launch_bounds__(kWave* kMaxWaves) void stats_kernel(StatArgs A)
 {
     __shared__ double red[kMaxWaves][5][kWave];
     const int lane = threadIdx.x & (kWave - 1);
@@ -248,7 +248,7 @@ __global__ __launch_bounds__(kWave* kMaxWaves) void stats_kernel(StatArgs A)
 //   tprof[7][z]    += plane mean of u w            (covariance + product)
 //   tprof[8][z]    += plane mean of rho^2
 // so the time-averaged rms is sqrt(<u^2> - <u>^2) over planes and time.
-__global__ __launch_bounds__(256) void tavg_profile_kernel(const double* __restrict__ means,
+static __global__ __launch_bounds__(256) void tavg_profile_kernel(const double* __restrict__ means,
                                                            const double* __restrict__ fl, int Z, double nxy,
                                                            double* __restrict__ tprof, DevState* st)
 {
@@ -273,7 +273,7 @@ __global__ __launch_bounds__(256) void tavg_profile_kernel(const double* __restr
 // the slip velocity; one thread per column, fixed-order block sums.
 //   out[q][blk], q = sum u over bottom-slip, bottom-noslip, top-slip, top-noslip
 template <typename T>
-__global__ __launch_bounds__(256) void slip_velocity_kernel(StatArgs A, double* __restrict__ out)
+static __global__ __launch_bounds__(256) void slip_velocity_kernel(StatArgs A, double* __restrict__ out)
 {
     __shared__ double sh[4][256];
     const int X = A.g.X, Y = A.g.Y, Z = A.g.Z;
@@ -353,7 +353,7 @@ __device__ __forceinline__ void feq_fp64(double u0, double u1, double u2, double
 //              potential built from Fourier modes kx, ky <= nmodes with a
 //              sin^2 wall-normal envelope (divergence-free, zero at walls).
 template <typename T>
-__global__ __launch_bounds__(256) void init_kernel(InitArgs A)
+static __global__ __launch_bounds__(256) void init_kernel(InitArgs A)
 {
     const int X = A.g.X, Y = A.g.Y, Z = A.g.Z;
     const long long total = (long long)X * Y * Z;

@@ -304,8 +304,14 @@ __host__ __device__ inline int step_waves(int Z)
     return w;
 }
 
-// The fused step: pull-stream + wall rule + BGK collision with force +
-// per-plane statistics + momentum budget of the next state.
+// Register accumulators of one wave over the columns it updates.
+struct StepAcc {
+    double jx = 0.0, mass = 0.0, mom = 0.0, sq = 0.0;
+};
+
+// The fused step of one column (all Z rows in 64-row chunks, one wave):
+// pull-stream + wall rule + BGK collision with force + per-plane
+// statistics + momentum budget of the next state.
 //   FIRST: the source holds post-stream populations (after a load or an
 //          init), so the gather is local instead of a pull.
 //   STATS: accumulate the per-plane sums for the per-step logs (in each
@@ -316,174 +322,162 @@ __host__ __device__ inline int step_waves(int Z)
 // that a halfway bounce-back reverses e_x:
 //     sum f_next = sum f,  sum e_x f_next = sum (j_x + G) - 2 sum_{BB} e_x g
 template <typename T, bool FIRST, bool STATS, bool BANDS>
-__global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs A)
+__device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane, double G, const double* shl,
+                                            double* mine, StepAcc& acc)
 {
-    extern __shared__ double lds[];  // see step_lds_bytes
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int W = blockDim.x / kWave;
     const int X = A.g.X, Y = A.g.Y, Z = A.g.Z, nzc = A.g.nzc;
     const int ctop = (Z - 1) / kWave, ltop = (Z - 1) % kWave;
     const T* __restrict__ src = (const T*)A.src;
     char* __restrict__ d8 = (char*)A.dst;
-    double* red = lds;  // [W][kNG][kWave]
-    double* planes = lds + (size_t)W * kNG * kWave;
-    double* mine = planes + (size_t)wave * kNQ * Z;
-    double* shl = planes + (size_t)W * kNQ * Z;
-
-    if constexpr (STATS) {
-        for (int i = lane; i < kNQ * Z; i += kWave) mine[i] = 0.0;
-        for (int i = threadIdx.x; i < 3 * Z; i += blockDim.x) shl[i] = A.shift[i];
-        __syncthreads();
+    const Column c = make_column(col, X, Y, Z);
+    const uint32_t ww = load_uniform_u32(A.wword + col);
+    // solid bands of the 3x3 neighbourhood, bw[ex+1][ey+1] = column (x - ex, y - ey)
+    uint32_t bw[3][3];
+    int maxhb = 0, maxht = 0;
+    if constexpr (BANDS) {
+        const int sx[3] = {c.xp, c.x, c.xm}, sy[3] = {c.yp, c.y, c.ym};
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                bw[i][j] = load_uniform_u32(A.bands + sx[i] * Y + sy[j]);
+                maxhb = max(maxhb, (int)(bw[i][j] & 0xFFFFu));
+                maxht = max(maxht, (int)(bw[i][j] >> 16));
+            }
     }
-    double gjx = 0.0, gmass = 0.0, gmom = 0.0, gsq = 0.0;
-    const double G = *A.G;
-
-    for (int it = blockIdx.x * W + wave; it < A.g.ncols; it += gridDim.x * W) {
-        const int col = __builtin_amdgcn_readfirstlane(it);
-        const Column c = make_column(col, X, Y, Z);
-        const uint32_t ww = load_uniform_u32(A.wword + col);
-        // solid bands of the 3x3 neighbourhood, bw[ex+1][ey+1] = column (x - ex, y - ey)
-        uint32_t bw[3][3];
-        int maxhb = 0, maxht = 0;
+    for (int ch = 0; ch < nzc; ch++) {
+        const int z = ch * kWave + lane;
+        if (z >= Z) break;
+        const bool bot_lane = ch == 0 && lane == 0;
+        const bool top_lane = ch == ctop && lane == ltop;
+        const uint32_t zoff = (uint32_t)z * (uint32_t)sizeof(T);
+        const uint32_t zoff_up = (uint32_t)(z == 0 ? 1 : z) * (uint32_t)sizeof(T);
+        const uint32_t zoff_dn = (uint32_t)(z == Z - 1 ? Z - 2 : z) * (uint32_t)sizeof(T);
+        T v[Q];
+        // chunk touches a band of the neighbourhood (upstream rows z-1..z+1)
+        bool slow = false, solid_own = false;
         if constexpr (BANDS) {
-            const int sx[3] = {c.xp, c.x, c.xm}, sy[3] = {c.yp, c.y, c.ym};
-#pragma unroll
-            for (int i = 0; i < 3; i++)
-#pragma unroll
-                for (int j = 0; j < 3; j++) {
-                    bw[i][j] = load_uniform_u32(A.bands + sx[i] * Y + sy[j]);
-                    maxhb = max(maxhb, (int)(bw[i][j] & 0xFFFFu));
-                    maxht = max(maxht, (int)(bw[i][j] >> 16));
-                }
+            const int z0 = ch * kWave, z1 = min(z0 + kWave, Z);
+            slow = (maxhb > 0 && z0 - 1 < maxhb) || (maxht > 0 && z1 >= Z - maxht);
+            solid_own = band_solid(bw[1][1], z, Z);
         }
-        for (int ch = 0; ch < nzc; ch++) {
-            const int z = ch * kWave + lane;
-            if (z >= Z) break;
-            const bool bot_lane = ch == 0 && lane == 0;
-            const bool top_lane = ch == ctop && lane == ltop;
-            const uint32_t zoff = (uint32_t)z * (uint32_t)sizeof(T);
-            const uint32_t zoff_up = (uint32_t)(z == 0 ? 1 : z) * (uint32_t)sizeof(T);
-            const uint32_t zoff_dn = (uint32_t)(z == Z - 1 ? Z - 2 : z) * (uint32_t)sizeof(T);
-            T v[Q];
-            // chunk touches a band of the neighbourhood (upstream rows z-1..z+1)
-            bool slow = false, solid_own = false;
-            if constexpr (BANDS) {
-                const int z0 = ch * kWave, z1 = min(z0 + kWave, Z);
-                slow = (maxhb > 0 && z0 - 1 < maxhb) || (maxht > 0 && z1 >= Z - maxht);
-                solid_own = band_solid(bw[1][1], z, Z);
-            }
-            if (BANDS && slow && !FIRST) {
-                // each lane selects its source before the load: halfway
-                // bounce-back from its own column when the upstream node is
-                // solid (band or no-slip wall row), the pull otherwise
-                const char* s8 = (const char*)src;
-                static_for<0, Q>([&](auto ac) {
-                    constexpr int a = decltype(ac)::value;
-                    const int zs = z - EZ[a];
-                    const bool solid_up =
-                        a != 0 && (zs < 0 || zs >= Z || band_solid(bw[EX[a] + 1][EY[a] + 1], zs, Z));
-                    const long long e = (long long)a * Z + colbase<EX[a], EY[a]>(c) - EZ[a];
-                    const uint32_t off = EZ[a] == 1 ? zoff_up : (EZ[a] == -1 ? zoff_dn : zoff);
-                    const char* pn = s8 + e * (long long)sizeof(T) + off;
-                    const char* pb = s8 + (c.cb + (long long)OPP[a] * Z) * (long long)sizeof(T) + zoff;
-                    v[a] = *(const T*)(solid_up ? pb : pn);
-                });
-            } else {
-                gather_wave<T, FIRST>(src, c, zoff, zoff_up, zoff_dn, bot_lane, top_lane, z, ww, v);
-            }
+        if (BANDS && slow && !FIRST) {
+            // each lane selects its source before the load: halfway
+            // bounce-back from its own column when the upstream node is
+            // solid (band or no-slip wall row), the pull otherwise
+            const char* s8 = (const char*)src;
+            static_for<0, Q>([&](auto ac) {
+                constexpr int a = decltype(ac)::value;
+                const int zs = z - EZ[a];
+                const bool solid_up =
+                    a != 0 && (zs < 0 || zs >= Z || band_solid(bw[EX[a] + 1][EY[a] + 1], zs, Z));
+                const long long e = (long long)a * Z + colbase<EX[a], EY[a]>(c) - EZ[a];
+                const uint32_t off = EZ[a] == 1 ? zoff_up : (EZ[a] == -1 ? zoff_dn : zoff);
+                const char* pn = s8 + e * (long long)sizeof(T) + off;
+                const char* pb = s8 + (c.cb + (long long)OPP[a] * Z) * (long long)sizeof(T) + zoff;
+                v[a] = *(const T*)(solid_up ? pb : pn);
+            });
+        } else {
+            gather_wave<T, FIRST>(src, c, zoff, zoff_up, zoff_dn, bot_lane, top_lane, z, ww, v);
+        }
 
-            double mass_node, jx_raw, g11, g12, g13, g14;
-            double gx[Q];  // post-collision values of the e_x != 0 directions (budget)
-            if constexpr (std::is_same<T, double>::value) {
-                double ux, uy, uz, dens;
-                moments_fp64(v, ux, uy, uz, dens);
-                const double u0 = ux / dens, u1 = uy / dens, u2 = uz / dens;
-                collide_fp64(v, u0, u1, u2, dens, A.omega, G);
-                jx_raw = ux;
-                mass_node = dens - 1.0;
-                if constexpr (STATS) {
-                    if (!solid_own) {
-                        const double du = u0 - shl[z], dv = u1 - shl[Z + z], dw = u2 - shl[2 * Z + z];
-                        mine[0 * Z + z] += du;
-                        mine[1 * Z + z] += dv;
-                        mine[2 * Z + z] += dw;
-                        mine[3 * Z + z] += mass_node;
-                        gsq += (du * du + dv * dv) + dw * dw;
-                    }
+        double mass_node, jx_raw, g11, g12, g13, g14;
+        double gx[Q];  // post-collision values of the e_x != 0 directions (budget)
+        if constexpr (std::is_same<T, double>::value) {
+            double ux, uy, uz, dens;
+            moments_fp64(v, ux, uy, uz, dens);
+            const double u0 = ux / dens, u1 = uy / dens, u2 = uz / dens;
+            collide_fp64(v, u0, u1, u2, dens, A.omega, G);
+            jx_raw = ux;
+            mass_node = dens - 1.0;
+            if constexpr (STATS) {
+                if (!solid_own) {
+                    const double du = u0 - shl[z], dv = u1 - shl[Z + z], dw = u2 - shl[2 * Z + z];
+                    mine[0 * Z + z] += du;
+                    mine[1 * Z + z] += dv;
+                    mine[2 * Z + z] += dw;
+                    mine[3 * Z + z] += mass_node;
+                    acc.sq += (du * du + dv * dv) + dw * dw;
                 }
+            }
 #pragma unroll
-                for (int a = 0; a < Q; a++) gx[a] = v[a];
-            } else {
-                float drho, u0, u1, u2, jx;
-                collide_fp32(v, (float)A.omega, (float)G, drho, u0, u1, u2, jx);
-                jx_raw = (double)jx;
-                mass_node = (double)drho;
-                if constexpr (STATS) {
-                    if (!solid_own) {
-                        const double du = (double)u0 - shl[z], dv = (double)u1 - shl[Z + z],
-                                     dw = (double)u2 - shl[2 * Z + z];
-                        mine[0 * Z + z] += du;
-                        mine[1 * Z + z] += dv;
-                        mine[2 * Z + z] += dw;
-                        mine[3 * Z + z] += mass_node;
-                        gsq += (du * du + dv * dv) + dw * dw;
-                    }
+            for (int a = 0; a < Q; a++) gx[a] = v[a];
+        } else {
+            float drho, u0, u1, u2, jx;
+            collide_fp32(v, (float)A.omega, (float)G, drho, u0, u1, u2, jx);
+            jx_raw = (double)jx;
+            mass_node = (double)drho;
+            if constexpr (STATS) {
+                if (!solid_own) {
+                    const double du = (double)u0 - shl[z], dv = (double)u1 - shl[Z + z],
+                                 dw = (double)u2 - shl[2 * Z + z];
+                    mine[0 * Z + z] += du;
+                    mine[1 * Z + z] += dv;
+                    mine[2 * Z + z] += dw;
+                    mine[3 * Z + z] += mass_node;
+                    acc.sq += (du * du + dv * dv) + dw * dw;
                 }
+            }
 #pragma unroll
-                for (int a = 0; a < Q; a++) gx[a] = WD[a] + (double)v[a];
-            }
-            g11 = gx[11];
-            g12 = gx[12];
-            g13 = gx[13];
-            g14 = gx[14];
-            double mom_node = jx_raw + G;
-            if (BANDS && slow) {
-                // every push of an e_x != 0 population into a solid node (band
-                // or no-slip wall row) bounces back and reverses e_x
-                static_for<1, Q>([&](auto ac) {
-                    constexpr int a = decltype(ac)::value;
-                    if constexpr (EX[a] != 0) {
-                        const int zt = z + EZ[a];
-                        const bool solid_t = zt < 0 || zt >= Z || band_solid(bw[1 - EX[a]][1 - EY[a]], zt, Z);
-                        if (solid_t) mom_node -= 2.0 * (double)EX[a] * gx[a];
-                    }
-                });
-            } else {
-                // bounce-back reverses e_x of the populations pushed into a
-                // no-slip wall node from a no-slip wall column (xcomputations.c:370)
-                if (bot_lane)
-                    mom_node = mom_node - (((ww >> 10) & 1u) ? 2.0 * g12 : 0.0) + (((ww >> 11) & 1u) ? 2.0 * g13 : 0.0);
-                if (top_lane)
-                    mom_node = mom_node - (((ww >> 12) & 1u) ? 2.0 * g11 : 0.0) + (((ww >> 13) & 1u) ? 2.0 * g14 : 0.0);
-            }
-            if (!solid_own) {
-                if constexpr (STATS) gjx += jx_raw;
-                gmass += mass_node;
-                gmom += mom_node;
-                static_for<0, Q>([&](auto ac) {
-                    constexpr int a = decltype(ac)::value;
-                    const long long e = c.cb + (long long)a * Z;
-                    *(T*)(d8 + e * (long long)sizeof(T) + zoff) = v[a];
-                });
-            }
+            for (int a = 0; a < Q; a++) gx[a] = WD[a] + (double)v[a];
+        }
+        g11 = gx[11];
+        g12 = gx[12];
+        g13 = gx[13];
+        g14 = gx[14];
+        double mom_node = jx_raw + G;
+        if (BANDS && slow) {
+            // every push of an e_x != 0 population into a solid node (band
+            // or no-slip wall row) bounces back and reverses e_x
+            static_for<1, Q>([&](auto ac) {
+                constexpr int a = decltype(ac)::value;
+                if constexpr (EX[a] != 0) {
+                    const int zt = z + EZ[a];
+                    const bool solid_t = zt < 0 || zt >= Z || band_solid(bw[1 - EX[a]][1 - EY[a]], zt, Z);
+                    if (solid_t) mom_node -= 2.0 * (double)EX[a] * gx[a];
+                }
+            });
+        } else {
+            // bounce-back reverses e_x of the populations pushed into a
+            // no-slip wall node from a no-slip wall column (xcomputations.c:370)
+            if (bot_lane)
+                mom_node = mom_node - (((ww >> 10) & 1u) ? 2.0 * g12 : 0.0) + (((ww >> 11) & 1u) ? 2.0 * g13 : 0.0);
+            if (top_lane)
+                mom_node = mom_node - (((ww >> 12) & 1u) ? 2.0 * g11 : 0.0) + (((ww >> 13) & 1u) ? 2.0 * g14 : 0.0);
+        }
+        if (!solid_own) {
+            if constexpr (STATS) acc.jx += jx_raw;
+            acc.mass += mass_node;
+            acc.mom += mom_node;
+            static_for<0, Q>([&](auto ac) {
+                constexpr int a = decltype(ac)::value;
+                const long long e = c.cb + (long long)a * Z;
+                *(T*)(d8 + e * (long long)sizeof(T) + zoff) = v[a];
+            });
         }
     }
+}
 
-    // Deterministic reductions: plane sums over the block's waves in wave
-    // order, global sums over waves then a fixed butterfly over lanes.
+// Deterministic reductions of one block: plane sums over the block's waves
+// in wave order, global sums over waves then a fixed butterfly over lanes;
+// written as partial number `slot` of [kNQ][Z][nslot] and [kNG][nslot].
+template <bool STATS>
+__device__ __forceinline__ void step_block_reduce(double* red, const double* planes, const StepAcc& acc, int Z,
+                                                  int W, int wave, int lane, double* part, double* gpart, int nslot,
+                                                  int slot)
+{
     double* myred = red + (size_t)wave * kNG * kWave;
-    myred[0 * kWave + lane] = gjx;
-    myred[1 * kWave + lane] = gmass;
-    myred[2 * kWave + lane] = gmom;
-    myred[3 * kWave + lane] = gsq;
+    myred[0 * kWave + lane] = acc.jx;
+    myred[1 * kWave + lane] = acc.mass;
+    myred[2 * kWave + lane] = acc.mom;
+    myred[3 * kWave + lane] = acc.sq;
     __syncthreads();
     if constexpr (STATS) {
         for (int i = threadIdx.x; i < kNQ * Z; i += blockDim.x) {
             double s = 0.0;
             for (int w = 0; w < W; w++) s += planes[(size_t)w * kNQ * Z + i];
             const int q = i / Z, z = i - q * Z;
-            A.part[((long long)q * Z + z) * A.nblk + blockIdx.x] = s;
+            part[((long long)q * Z + z) * nslot + slot] = s;
         }
     }
     if (wave == 0) {
@@ -494,14 +488,41 @@ __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs A)
             for (int w = 0; w < W; w++) s += red[((size_t)w * kNG + q) * kWave + lane];
 #pragma unroll
             for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
-            if (lane == 0) A.gpart[(long long)q * A.nblk + blockIdx.x] = s;
+            if (lane == 0) gpart[(long long)q * nslot + slot] = s;
         }
     }
 }
 
+// The single-case step: the block's waves take columns round-robin over
+// the grid; one set of partials per block.
+template <typename T, bool FIRST, bool STATS, bool BANDS>
+static __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs A)
+{
+    extern __shared__ double lds[];  // see step_lds_bytes
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+    const int W = blockDim.x / kWave;
+    const int Z = A.g.Z;
+    double* red = lds;  // [W][kNG][kWave]
+    double* planes = lds + (size_t)W * kNG * kWave;
+    double* mine = planes + (size_t)wave * kNQ * Z;
+    double* shl = planes + (size_t)W * kNQ * Z;
+
+    if constexpr (STATS) {
+        for (int i = lane; i < kNQ * Z; i += kWave) mine[i] = 0.0;
+        for (int i = threadIdx.x; i < 3 * Z; i += blockDim.x) shl[i] = A.shift[i];
+        __syncthreads();
+    }
+    StepAcc acc;
+    const double G = *A.G;
+    for (int it = blockIdx.x * W + wave; it < A.g.ncols; it += gridDim.x * W)
+        step_column<T, FIRST, STATS, BANDS>(A, __builtin_amdgcn_readfirstlane(it), lane, G, shl, mine, acc);
+    step_block_reduce<STATS>(red, planes, acc, Z, W, wave, lane, A.part, A.gpart, A.nblk, blockIdx.x);
+}
+
 // Per-column wall words from the wall masks (the pull table of the push
 // rule xcomputations.c:368-375, and its bounce-back pushes).
-__global__ __launch_bounds__(256) void wall_word_kernel(const int8_t* __restrict__ mbot,
+static __global__ __launch_bounds__(256) void wall_word_kernel(const int8_t* __restrict__ mbot,
                                                         const int8_t* __restrict__ mtop, int X, int Y,
                                                         uint32_t* __restrict__ ww)
 {
@@ -562,7 +583,7 @@ __device__ __forceinline__ void block_reduce_write(const StepArgs& A, double (&a
 // the same accumulators as the step, used after a load/init to derive the
 // first step's force (encalc.c:121 on the initial state) and the shifts.
 template <typename T, bool LOCAL>
-__global__ __launch_bounds__(kWave* kMaxWaves) void moments_kernel(StepArgs A)
+static __global__ __launch_bounds__(kWave* kMaxWaves) void moments_kernel(StepArgs A)
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
@@ -620,7 +641,7 @@ __global__ __launch_bounds__(kWave* kMaxWaves) void moments_kernel(StepArgs A)
 
 // Fixed-order reduction of the partials: block b sums row b of a
 // [rows][nblk] matrix.
-__global__ __launch_bounds__(256) void reduce_rows_kernel(const double* __restrict__ in, int nblk,
+static __global__ __launch_bounds__(256) void reduce_rows_kernel(const double* __restrict__ in, int nblk,
                                                           double* __restrict__ out)
 {
     __shared__ double sh[256];
@@ -643,7 +664,8 @@ struct DevState {
     // FP64 time accumulators of the logged rows since the last reset
     double n_rows, s_pgrad, s_stress, s_stup, s_stdown, s_U, s_K, s_rho;
     double n_prof;  // profile samples accumulated (tavg_profile_kernel)
-    double pad[7];
+    double steps;   // step counter: the counter of the next step's log row
+    double pad[6];
 };
 
 struct LogRow {  // mirrors dns_steplog
@@ -663,15 +685,24 @@ struct FinalizeArgs {
     int from_moments;       // 1: globals came from moments_kernel (initial state)
     const double* nf;       // [Z] fluid nodes per plane (null: X*Y, flat walls)
     double nfluid;          // fluid nodes in total
+    int cntr_dev;           // 1: the row counter is DevState::steps instead of cntr
 };
 
 // Per-step finisher (encalc.c:29-138 and wshearstr.c:26-154 restated on
 // plane sums): the log row of this step, the next step's force and the
 // new plane-mean shifts.  One block; thread z owns plane z, sums over z in
 // a fixed tree.
-__global__ __launch_bounds__(1024) void finalize_kernel(FinalizeArgs A)
+// Scratch of the finisher: four arrays of blockDim.x (<= 1024) doubles.
+struct FinalizeShared {
+    double sU[1024], sK[1024], uav[1024], dav[1024];
+};
+
+__device__ __forceinline__ void finalize_body(const FinalizeArgs& A, FinalizeShared& S)
 {
-    __shared__ double sU[1024], sK[1024], uav[1024], dav[1024];
+    double* sU = S.sU;
+    double* sK = S.sK;
+    double* uav = S.uav;
+    double* dav = S.dav;
     const int Z = A.Z;
     const double Nxy = (double)A.X * (double)A.Y;
     const double N = Nxy * (double)Z;
@@ -718,8 +749,9 @@ __global__ __launch_bounds__(1024) void finalize_kernel(FinalizeArgs A)
             const double U = sU[0] / (double)Z;
             const double stdown = (-2. * uav[0] + 3. * uav[1] - uav[2]) * (A.mu * dav[0]);
             const double stup = -(2. * uav[Z - 1] - 3. * uav[Z - 2] + uav[Z - 3]) * (A.mu * dav[Z - 3]);
+            const long long cntr = A.cntr_dev ? (long long)A.st->steps : A.cntr;
             LogRow r;
-            r.t = A.TM + A.dt * (double)A.cntr;
+            r.t = A.TM + A.dt * (double)cntr;
             r.K = K;
             r.flowrate = U * (double)(A.Y * Z);
             r.massrate = momentum * (double)(A.Y * Z);
@@ -740,6 +772,7 @@ __global__ __launch_bounds__(1024) void finalize_kernel(FinalizeArgs A)
             A.st->s_K += K;
             A.st->s_rho += denst;
         }
+        if (!A.from_moments) A.st->steps = (A.cntr_dev ? A.st->steps : (double)A.cntr) + 1.0;
         // force of the next step (encalc.c:121 on the next state)
         if (A.forcing == 0) {
             double rho_next, mom_next;
@@ -757,6 +790,12 @@ __global__ __launch_bounds__(1024) void finalize_kernel(FinalizeArgs A)
             A.st->G = A.F;
         }
     }
+}
+
+static __global__ __launch_bounds__(1024) void finalize_kernel(FinalizeArgs A)
+{
+    __shared__ FinalizeShared S;
+    finalize_body(A, S);
 }
 
 }  // namespace lbm

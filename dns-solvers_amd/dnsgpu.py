@@ -27,7 +27,13 @@ EXPORTS = (
     "dnsgpu_commit_load", "dnsgpu_save_aos", "dnsgpu_get_force", "dnsgpu_step",
     "dnsgpu_statistics", "dnsgpu_fields", "dnsgpu_time_steps", "dnsgpu_device_bytes",
     "dnsgpu_autotune", "dnsgpu_tavg_reset", "dnsgpu_tavg_sample", "dnsgpu_tavg_get",
+    "dnsgpu_footprint",
+    "dnsens_create", "dnsens_destroy", "dnsens_member_bytes", "dnsens_add", "dnsens_remove",
+    "dnsens_count", "dnsens_bytes_used", "dnsens_budget", "dnsens_step", "dnsens_info", "dnsens_rows",
+    "dnsens_autotune", "dnsens_time_steps", "dnsens_grid",
 )
+
+ENS_NO_GRAPH, ENS_AUTOTUNE = 1, 2
 
 TAVG_FIELDS = ("nsteps", "nsamples", "pgrad", "utau", "stress", "stup", "stdown", "Ub", "K", "rho", "Cf_w",
                "Cf_p", "Re_tau", "u_slip_bottom", "u_noslip_bottom", "u_slip_top", "u_noslip_top")
@@ -67,6 +73,12 @@ class DnsStepLog(ctypes.Structure):
 class DnsInit(ctypes.Structure):
     _fields_ = [("kind", ctypes.c_int), ("uc", ctypes.c_double), ("amp", ctypes.c_double),
                 ("seed", ctypes.c_uint64), ("nmodes", ctypes.c_int)]
+
+
+class DnsEnsInfo(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_double), ("n_rows", ctypes.c_double), ("mean_pgrad", ctypes.c_double),
+                ("force", ctypes.c_double), ("last_stress", ctypes.c_double), ("finite", ctypes.c_int),
+                ("pad", ctypes.c_int)]
 
 
 class DnsGpuError(RuntimeError):
@@ -115,6 +127,26 @@ def load():
         L.dnsgpu_tavg_get.argtypes = [vp, ctypes.POINTER(DnsTavg), dp]
         L.dnsgpu_device_bytes.argtypes = [vp]
         L.dnsgpu_device_bytes.restype = ctypes.c_size_t
+        L.dnsgpu_footprint.argtypes = [ctypes.POINTER(DnsCase)]
+        L.dnsgpu_footprint.restype = ctypes.c_size_t
+        L.dnsens_create.argtypes = [ctypes.c_int, ctypes.c_size_t, ctypes.c_int, ctypes.POINTER(vp)]
+        L.dnsens_destroy.argtypes = [vp]
+        L.dnsens_destroy.restype = None
+        L.dnsens_member_bytes.argtypes = [vp]
+        L.dnsens_member_bytes.restype = ctypes.c_size_t
+        L.dnsens_add.argtypes = [vp, vp]
+        L.dnsens_remove.argtypes = [vp, vp]
+        L.dnsens_count.argtypes = [vp]
+        L.dnsens_bytes_used.argtypes = [vp]
+        L.dnsens_bytes_used.restype = ctypes.c_size_t
+        L.dnsens_budget.argtypes = [vp]
+        L.dnsens_budget.restype = ctypes.c_size_t
+        L.dnsens_step.argtypes = [vp, ctypes.c_int]
+        L.dnsens_info.argtypes = [vp, vp, ctypes.POINTER(DnsEnsInfo)]
+        L.dnsens_rows.argtypes = [vp, vp, ctypes.POINTER(DnsStepLog), ctypes.c_int]
+        L.dnsens_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), dp]
+        L.dnsens_time_steps.argtypes = [vp, ctypes.c_int, dp]
+        L.dnsens_grid.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         _lib = L
     return _lib
 
@@ -279,3 +311,92 @@ class Channel:
 
     def device_bytes(self):
         return load().dnsgpu_device_bytes(self.h)
+
+
+def footprint(xdv, ydv, zdv, precision=FP64):
+    """Device bytes a handle of this grid takes (0: refused)."""
+    c = DnsCase(xdv, ydv, zdv, 100.0, 0.1, 0.1, 0, CONST_Q, 0.0, precision, 0.0)
+    return load().dnsgpu_footprint(ctypes.byref(c))
+
+
+def _rows_dict(rows, n):
+    return {k: np.array([getattr(rows[i], k) for i in range(n)]) for k in LOG_FIELDS}
+
+
+class Ensemble:
+    """Resident cases advancing together (a dnsens pool); members are
+    Channel objects."""
+
+    def __init__(self, precision=FP32, budget=0, flags=0):
+        self.e = ctypes.c_void_p()
+        _chk(load().dnsens_create(precision, budget, flags, ctypes.byref(self.e)))
+        self.members = []
+
+    def close(self):
+        if self.e:
+            load().dnsens_destroy(self.e)
+            self.e = ctypes.c_void_p()
+            self.members = []
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def add(self, ch):
+        _chk(load().dnsens_add(self.e, ch.h))
+        self.members.append(ch)
+
+    def remove(self, ch):
+        _chk(load().dnsens_remove(self.e, ch.h))
+        self.members.remove(ch)
+
+    def count(self):
+        return load().dnsens_count(self.e)
+
+    def bytes_used(self):
+        return load().dnsens_bytes_used(self.e)
+
+    def budget(self):
+        return load().dnsens_budget(self.e)
+
+    @staticmethod
+    def member_bytes(ch):
+        return load().dnsens_member_bytes(ch.h)
+
+    def step(self, n):
+        _chk(load().dnsens_step(self.e, n))
+
+    def info(self, ch):
+        i = DnsEnsInfo()
+        _chk(load().dnsens_info(self.e, ch.h, ctypes.byref(i)))
+        return {k: getattr(i, k) for k, _ in DnsEnsInfo._fields_ if k != "pad"}
+
+    def rows(self, ch, n):
+        rows = (DnsStepLog * max(n, 1))()
+        _chk(load().dnsens_rows(self.e, ch.h, rows, n))
+        return _rows_dict(rows, n)
+
+    def autotune(self, trial_steps=3):
+        b = ctypes.c_int()
+        t = ctypes.c_double()
+        _chk(load().dnsens_autotune(self.e, trial_steps, ctypes.byref(b), ctypes.byref(t)))
+        return b.value, t.value
+
+    def time_steps(self, n):
+        t = ctypes.c_double()
+        _chk(load().dnsens_time_steps(self.e, n, ctypes.byref(t)))
+        return t.value
+
+    def grid(self):
+        b = ctypes.c_int()
+        t = ctypes.c_int()
+        _chk(load().dnsens_grid(self.e, ctypes.byref(b), ctypes.byref(t)))
+        return b.value, t.value

@@ -213,6 +213,89 @@ int dnsgpu_autotune(dnsgpu *h, int trial_steps, int *best_blocks, double *best_m
 /* Device bytes held by the handle. */
 size_t dnsgpu_device_bytes(const dnsgpu *h);
 
+/* Device bytes dnsgpu_create would allocate for this case (0 when the case
+ * is refused), without allocating: the admission test of a request. */
+size_t dnsgpu_footprint(const dns_case *c);
+
+/* ------------------------------------------------------------ ensembles
+ *
+ * An ensemble advances many resident cases (members) of different grids,
+ * textures and forcing together: per time step one launch of a multi-case
+ * step kernel over a table of column tiles of every member, one launch of
+ * the partial-sum reduction and one of the per-member finisher (log row,
+ * next force), captured into a HIP graph per chunk of steps.  No
+ * counterpart in the reference, which runs one case per MPI job; each
+ * member computes exactly the reference's step (the same kernels' column
+ * update), and its reductions run in an order fixed by the member alone,
+ * so a member's results are the same alone and in any mix.
+ *
+ * A member stays a dnsgpu handle: its logs go to its time accumulators
+ * (dnsgpu_tavg_*), and dnsgpu_save_aos, _statistics, _fields, _tavg_* work
+ * on it between dnsens_step calls.  Entry points that change or advance
+ * its state (dnsgpu_step, _load_aos, _init_*, _set_walls, ...) return
+ * DNSGPU_EARG while it is a member; dnsgpu_destroy removes it first.
+ */
+typedef struct dnsens dnsens;
+
+#define DNSENS_MAX_MEMBERS 256
+#define DNSENS_NO_GRAPH 1 /* flags: launch the kernels directly instead of replaying a graph */
+#define DNSENS_AUTOTUNE 2 /* flags: tune the step grid whenever the member set takes a new shape */
+
+/* precision: DNSGPU_FP64 or DNSGPU_FP32, that of every member.
+ * hbm_budget: admission limit on the sum of the members' device bytes
+ * (0: 90% of the device memory free now). */
+int dnsens_create(int precision, size_t hbm_budget, int flags, dnsens **out);
+void dnsens_destroy(dnsens *e);
+
+/* Device bytes a handle is charged as a member (its own plus any tile
+ * partials the ensemble adds for it). */
+size_t dnsens_member_bytes(const dnsgpu *h);
+
+/* Admit handle h (initialised or loaded, same precision, zdv <= 377).
+ * Returns DNSGPU_ENOMEM, with nothing changed, when its bytes do not fit
+ * the budget.  A state fresh from an init or a load takes its first step
+ * here (one logged step). */
+int dnsens_add(dnsens *e, dnsgpu *h);
+/* The handle leaves the ensemble (state as of the last dnsens_step). */
+int dnsens_remove(dnsens *e, dnsgpu *h);
+int dnsens_count(const dnsens *e);
+size_t dnsens_bytes_used(const dnsens *e);
+size_t dnsens_budget(const dnsens *e);
+
+/* Advance every member by nsteps logged steps; returns when they are done.
+ * A member whose force or log row turned NaN is flagged in its info (and
+ * keeps running until removed). */
+int dnsens_step(dnsens *e, int nsteps);
+
+typedef struct dnsens_member_info {
+    double steps;       /* the member's step counter (counter of its next log row) */
+    double n_rows;      /* rows in its time accumulators since dnsgpu_tavg_reset */
+    double mean_pgrad;  /* running mean of p-grad over those rows */
+    double force;       /* force of its next step */
+    double last_stress; /* wall stress of its last step */
+    int finite;         /* 0 once its force or a log row was not finite */
+    int pad;
+} dnsens_member_info;
+
+/* State of member h after the last dnsens_step. */
+int dnsens_info(dnsens *e, const dnsgpu *h, dnsens_member_info *out);
+
+/* The last n log rows of member h from the last chunk of the last
+ * dnsens_step (a chunk holds up to 1024 steps). */
+int dnsens_rows(dnsens *e, const dnsgpu *h, dns_steplog *rows, int n);
+
+/* Choose the step kernel's grid for the current member set by timing
+ * trial_steps launches per candidate (state not advanced); cached per
+ * (precision, largest zdv, tile count) in the process. */
+int dnsens_autotune(dnsens *e, int trial_steps, int *best_blocks, double *best_ms);
+
+/* Advance nsteps steps like dnsens_step and return the elapsed device time
+ * (HIP events on the ensemble's stream) in ms. */
+int dnsens_time_steps(dnsens *e, int nsteps, double *ms);
+
+/* Current step-kernel grid (blocks) and tile count. */
+int dnsens_grid(const dnsens *e, int *blocks, int *tiles);
+
 #ifdef __cplusplus
 }
 #endif

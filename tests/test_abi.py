@@ -11,7 +11,7 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 def declared_symbols():
     txt = open(os.path.join(REPO, "include", "dnsgpu.h")).read()
-    return sorted(set(re.findall(r"\b(dnsgpu_[a-z_0-9]+)\s*\(", txt)))
+    return sorted(set(re.findall(r"\b((?:dnsgpu|dnsens)_[a-z_0-9]+)\s*\(", txt)))
 
 
 def test_header_lists_every_binding():
@@ -30,3 +30,4 @@ def test_struct_layouts_match_header():
     assert ctypes.sizeof(dnsgpu.DnsStepLog) == 80
     assert ctypes.sizeof(dnsgpu.DnsInit) == 40
     assert ctypes.sizeof(dnsgpu.DnsTavg) == 17 * 8
+    assert ctypes.sizeof(dnsgpu.DnsEnsInfo) == 5 * 8 + 8
