@@ -1,0 +1,74 @@
+"""The continuous-batching service: every solve's result is the same when it
+is served alone and in a mix that arrives over time (bitwise: the member's
+state does not depend on the others, and its convergence decisions use its
+own statistics only)."""
+import pytest
+
+import dnsgpu
+import dnsserve
+
+pytestmark = pytest.mark.gpu
+
+# grid Reynolds number Re_b / zdv at most about 70 (resolved enough for BGK
+# to stay bounded on these coarse grids)
+SMALL = [
+    dict(xdv=32, ydv=16, zdv=24, retau=55.0, texture=dnsserve.GROOVES, period=8, param=0.5, seed=1),
+    dict(xdv=16, ydv=32, zdv=40, retau=90.0, texture=dnsserve.RIBLETS, period=16, param=4.0, seed=2),
+    dict(xdv=24, ydv=24, zdv=70, retau=120.0, texture=dnsserve.GROOVES, period=12, param=0.75, seed=3),
+    dict(xdv=16, ydv=16, zdv=32, retau=70.0, texture=dnsserve.SMOOTH, period=0, param=0.0, seed=4),
+    dict(xdv=40, ydv=16, zdv=24, retau=60.0, texture=dnsserve.RIBLETS, period=8, param=5.0, seed=5),
+]
+
+
+def cfg(**kw):
+    base = dict(transient=300, window=400, check=100, cap=1500, tol=0.01, realtime=0, init_modes=4)
+    base.update(kw)
+    return dnsserve.defaults(**base)
+
+
+KEYS = ("steps", "status", "utau", "Re_tau", "Cf_w", "Cf_p", "Ub", "stress", "u_slip")
+
+
+def test_each_result_is_the_same_alone_and_in_the_mix():
+    print("case: serve small mix, burst then realtime", flush=True)
+    mix, summ = dnsserve.sweep(dnsserve.requests(SMALL), cfg())
+    assert summ["completed"] == len(SMALL) and summ["max_resident"] == len(SMALL), [r["status"] for r in mix]
+    for r in mix:
+        assert r["status"] in (0, 1) and r["steps"] >= 301 and r["utau"] > 0
+    for i, item in enumerate(SMALL):
+        alone, _ = dnsserve.sweep(dnsserve.requests([item]), cfg())
+        for k in KEYS:
+            assert alone[0][k] == mix[i][k], (i, k)
+    # arrivals over time: requests join while others run
+    timed = [dict(item, arrival=0.02 * i) for i, item in enumerate(SMALL)]
+    late, s2 = dnsserve.sweep(dnsserve.requests(timed), cfg(realtime=1))
+    assert s2["completed"] == len(SMALL)
+    for i in range(len(SMALL)):
+        for k in KEYS:
+            assert late[i][k] == mix[i][k], (i, k)
+        assert late[i]["finished"] >= late[i]["admitted"] >= late[i]["arrival"]
+    assert s2["p99"] >= s2["p50"] > 0
+
+
+def test_admission_waits_for_memory():
+    print("case: serve with a budget of one solve", flush=True)
+    # three solves of one grid: a budget of 1.5 footprints holds one at a time
+    items = [dict(SMALL[0], seed=11), dict(SMALL[0], texture=dnsserve.RIBLETS, period=8, param=4.0, seed=12),
+             dict(SMALL[0], param=0.25, seed=13)]
+    need = dnsgpu.footprint(32, 16, 24, dnsgpu.FP32)
+    res, summ = dnsserve.sweep(dnsserve.requests(items), cfg(budget=need + need // 2))
+    assert summ["max_resident"] == 1 and summ["completed"] == 3
+    # served one at a time, in arrival order
+    assert res[0]["finished"] <= res[1]["admitted"] and res[1]["finished"] <= res[2]["admitted"]
+    ref, _ = dnsserve.sweep(dnsserve.requests(items), cfg())
+    for a, b in zip(res, ref):
+        for k in KEYS:
+            assert a[k] == b[k], k
+
+
+def test_sequential_single_case_path_runs_the_same_steps():
+    print("case: sequential single-case path", flush=True)
+    items = SMALL[:2]
+    res, _ = dnsserve.sweep(dnsserve.requests(items), cfg())
+    secs = dnsserve.sequential(dnsserve.requests(items), cfg(), dnsgpu.FP64, [r["steps"] for r in res])
+    assert all(s > 0 for s in secs)
