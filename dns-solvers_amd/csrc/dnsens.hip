@@ -24,11 +24,9 @@ namespace {
 
 struct Member {
     dnsgpu* h;
-    size_t bytes;    // footprint charged: the handle plus any partials of its own
+    size_t bytes;  // footprint charged: the handle plus its tile records
     int ntiles;
-    double* part;    // [kNQ][Z][ntiles] (the handle's buffer when it is large enough)
-    double* gpart;   // [kNG][ntiles]
-    bool own_part;   // part/gpart allocated by the ensemble
+    double* rec;   // tile records [ntiles][ens_record(Z)]
     dnsens_member_info info;
 };
 
@@ -63,7 +61,7 @@ struct dnsens {
     std::vector<EnsMember> h_mem;
     std::vector<FinalizeArgs> h_fin;
     std::vector<EnsTile> h_tiles;
-    int ntiles, nrows, zmax;
+    int ntiles, nchunks, zmax;
     size_t lds;
     int resident;       // step-kernel blocks resident at once for this lds
     int blocks_per_cu;  // grid = blocks_per_cu * ncu (capped)
@@ -131,11 +129,8 @@ static void drop_graph(dnsens* e)
 
 static void free_member(Member& m)
 {
-    if (m.own_part) {
-        if (m.part) (void)hipFree(m.part);
-        if (m.gpart) (void)hipFree(m.gpart);
-    }
-    m.part = m.gpart = nullptr;
+    if (m.rec) (void)hipFree(m.rec);
+    m.rec = nullptr;
 }
 
 extern "C" void dnsens_destroy(dnsens* e)
@@ -163,21 +158,26 @@ static int find_member(const dnsens* e, const dnsgpu* h)
     return -1;
 }
 
-static int tiles_of(const dnsgpu* h) { return (h->g.ncols + kEnsTileCols - 1) / kEnsTileCols; }
-
-// Bytes a member adds on top of its handle (tile partials that do not fit
-// the handle's own partial buffers).
-static size_t extra_bytes(const dnsgpu* h)
+static int tiles_of(const dnsgpu* h)
 {
-    const size_t nt = (size_t)tiles_of(h);
-    const size_t have = (size_t)std::max(h->nblk, h->step_nblk_max);
-    if (nt <= have) return 0;
-    return sizeof(double) * ((size_t)kNQ * h->g.Z * nt + (size_t)kNG * nt);
+    const int cpt = ens_tile_cols(h->g.ncols);
+    return (h->g.ncols + cpt - 1) / cpt;
 }
+
+// Bytes a member adds on top of its handle: its tile records.
+static size_t extra_bytes(const dnsgpu* h) { return sizeof(double) * (size_t)tiles_of(h) * ens_record(h->g.Z); }
 
 extern "C" size_t dnsens_member_bytes(const dnsgpu* h)
 {
     return h ? h->dev_bytes + extra_bytes(h) : 0;
+}
+
+extern "C" size_t dnsens_footprint(const dns_case* c)
+{
+    const size_t fp = dnsgpu_footprint(c);
+    if (!fp) return 0;
+    const int ncols = c->xdv * c->ydv, cpt = ens_tile_cols(ncols);
+    return fp + sizeof(double) * (size_t)((ncols + cpt - 1) / cpt) * ens_record(c->zdv);
 }
 
 extern "C" int dnsens_add(dnsens* e, dnsgpu* h)
@@ -186,7 +186,6 @@ extern "C" int dnsens_add(dnsens* e, dnsgpu* h)
     if (h->c.precision != e->precision)
         return set_err(DNSGPU_EARG, "member precision %d differs from the ensemble's %d", h->c.precision,
                        e->precision);
-    if (h->g.Z > ens_max_z()) return set_err(DNSGPU_EARG, "zdv %d above the ensemble limit %d", h->g.Z, ens_max_z());
     if (find_member(e, h) >= 0) return set_err(DNSGPU_EARG, "handle is already a member");
     if ((int)e->mem.size() >= DNSENS_MAX_MEMBERS)
         return set_err(DNSGPU_EARG, "ensemble holds %d members already", DNSENS_MAX_MEMBERS);
@@ -200,16 +199,9 @@ extern "C" int dnsens_add(dnsens* e, dnsgpu* h)
     m.h = h;
     m.bytes = need;
     m.ntiles = tiles_of(h);
-    if (extra_bytes(h)) {
-        m.own_part = true;
-        if (hipMalloc(&m.part, sizeof(double) * kNQ * h->g.Z * (size_t)m.ntiles) != hipSuccess ||
-            hipMalloc(&m.gpart, sizeof(double) * kNG * (size_t)m.ntiles) != hipSuccess) {
-            free_member(m);
-            return set_err(DNSGPU_ENOMEM, "tile partials of %d tiles", m.ntiles);
-        }
-    } else {
-        m.part = h->part;
-        m.gpart = h->gpart;
+    if (hipMalloc(&m.rec, extra_bytes(h)) != hipSuccess) {
+        free_member(m);
+        return set_err(DNSGPU_ENOMEM, "tile records of %d tiles", m.ntiles);
     }
     // a state fresh from a load or an init holds post-stream populations:
     // its first step (local gather) runs through the single-case kernels
@@ -261,7 +253,8 @@ static int step_occupancy(size_t lds, int* per_cu)
 
 static int pick_grid(dnsens* e)
 {
-    e->grid = std::max(1, std::min(e->blocks_per_cu * e->ncu, std::min(e->resident, e->ntiles)));
+    const int wave_blocks = (e->ntiles + kEnsWaves - 1) / kEnsWaves;  // one wave per tile
+    e->grid = std::max(1, std::min(e->blocks_per_cu * e->ncu, std::min(e->resident, wave_blocks)));
     return 0;
 }
 
@@ -273,33 +266,34 @@ static int upload_tables(dnsens* e)
     e->h_mem.resize(nm);
     e->h_fin.resize(nm);
     e->h_tiles.clear();
-    int row = 0, zmax = 0;
+    int chunk = 0, zmax = 0;
     for (int i = 0; i < nm; i++) {
         Member& m = e->mem[i];
         dnsgpu* h = m.h;
         EnsMember& d = e->h_mem[i];
         d.a = step_args(h, h->buf[h->cur], h->buf[1 - h->cur], m.ntiles);
-        d.a.part = m.part;
-        d.a.gpart = m.gpart;
+        d.a.part = m.rec;
+        d.a.gpart = nullptr;
         d.planes = h->planes;
-        d.row0 = row;
-        d.nrows = kNQ * h->g.Z + kNG;
-        row += d.nrows;
+        d.chunk0 = chunk;
+        d.nchunks = (ens_record(h->g.Z) + kWave - 1) / kWave;
+        chunk += d.nchunks;
+        const int cpt = ens_tile_cols(h->g.ncols);
         FinalizeArgs F = fin_args(h, true, h->ring, 0, 0);
         F.cntr_dev = 1;
         e->h_fin[i] = F;
         for (int t = 0; t < m.ntiles; t++) {
             EnsTile tl;
             tl.m = i;
-            tl.col0 = t * kEnsTileCols;
-            tl.ncol = std::min(kEnsTileCols, h->g.ncols - tl.col0);
+            tl.col0 = t * cpt;
+            tl.ncol = std::min(cpt, h->g.ncols - tl.col0);
             tl.lt = t;
             e->h_tiles.push_back(tl);
         }
         zmax = std::max(zmax, h->g.Z);
     }
     e->ntiles = (int)e->h_tiles.size();
-    e->nrows = row;
+    e->nchunks = chunk;
     if (nm > e->cap_mem) {
         if (e->d_mem) (void)hipFree(e->d_mem);
         if (e->d_fin) (void)hipFree(e->d_fin);
@@ -337,7 +331,7 @@ static int upload_tables(dnsens* e)
     }
     if (zmax != e->zmax || e->resident == 0) {
         e->zmax = zmax;
-        e->lds = step_lds_bytes(std::max(zmax, 1), kEnsWaves, true);
+        e->lds = ens_lds_bytes(std::max(zmax, 1));
         int per = 0;
         int rc = e->precision == DNSGPU_FP64 ? step_occupancy<double>(e->lds, &per) : step_occupancy<float>(e->lds, &per);
         if (rc) return rc;
@@ -361,7 +355,7 @@ template <typename T>
 static void enqueue_step_only(dnsens* e, int phase)
 {
     hipLaunchKernelGGL(ens_step_kernel<T>, dim3(e->grid), dim3(kWave * kEnsWaves), e->lds, e->stream, e->d_mem,
-                       e->d_tiles, e->ntiles, phase);
+                       e->d_tiles, e->ntiles, std::max(e->zmax, 1), phase);
 }
 
 // k steps of every member: step, reduction, finisher per step; log row of
@@ -372,7 +366,8 @@ static int enqueue_steps(dnsens* e, int k)
     const int nm = (int)e->mem.size();
     for (int s = 0; s < k; s++) {
         enqueue_step_only<T>(e, s & 1);
-        hipLaunchKernelGGL(ens_reduce_kernel, dim3(e->nrows), dim3(256), 0, e->stream, e->d_mem, nm);
+        hipLaunchKernelGGL(ens_reduce_kernel, dim3(e->nchunks), dim3(kWave * kEnsReduceWaves), 0, e->stream, e->d_mem,
+                           nm);
         hipLaunchKernelGGL(ens_finalize_kernel, dim3(nm), dim3(1024), 0, e->stream, e->d_fin, s);
     }
     HIPCHK(hipGetLastError());

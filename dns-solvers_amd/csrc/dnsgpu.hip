@@ -516,6 +516,10 @@ extern "C" int dnsgpu_set_solid_bands(dnsgpu* h, const int16_t* bottom, const in
 {
     if (!h) return set_err(DNSGPU_EARG, "null handle");
     NOT_MEMBER(h);
+    // the step keeps the bounced-back populations in the solid nodes' slots,
+    // which a state advanced without these bands does not have
+    if (!h->poststream)
+        return set_err(DNSGPU_EARG, "solid bands are set before the first step (after create, an init or a load)");
     const int X = h->g.X, Y = h->g.Y, Z = h->g.Z;
     const size_t xy = (size_t)X * Y;
     std::vector<uint32_t> bw(xy, 0u);

@@ -3,43 +3,46 @@
 // numbers advance together, one launch of each kernel per step for all of
 // them.
 //
-// Work is described by a tile table: a tile is kEnsTileCols consecutive
-// columns of one member.  The step kernel's grid is sized to what the
-// device holds at once and loops over the table; each tile writes its own
-// set of partial sums, so every member's reductions run in an order fixed
-// by the member alone (tile count, then the fixed-order row reduction):
-// a member's results do not depend on which other members share the
-// launches, nor on the grid.
+// Work is described by a tile table: a tile is a few consecutive columns
+// of one member, updated by one wave.  The step kernel's grid is sized to
+// what the device holds at once and its waves loop over the table with no
+// block-level synchronisation; each tile writes one contiguous record of
+// its per-plane and global sums, so every member's reductions run in an
+// order fixed by the member alone (its tiles, then the fixed-order record
+// reduction): a member's results do not depend on which other members
+// share the launches, nor on the grid.
 #pragma once
 
 #include "lbm_kernels.h"
 
 namespace lbm {
 
-constexpr int kEnsTileCols = 16;  // columns per tile
-constexpr int kEnsWaves = 4;      // waves per block of the ensemble step kernel
+constexpr int kEnsWaves = 4;  // waves per block of the ensemble step kernel
 
-// Largest Z a member may have: the block's per-wave plane sums, global
-// partials and shifts in 64 KiB of LDS with kEnsWaves waves.
-constexpr int ens_max_z()
-{
-    return (65536 / (int)sizeof(double) - kEnsWaves * kNG * kWave) / (kEnsWaves * kNQ + 3);
-}
+// Columns per tile of a member: a function of the member alone.
+__host__ __device__ inline int ens_tile_cols(int ncols) { return ncols >= 32768 ? 16 : 8; }
+
+// Doubles per tile record: kNQ per-plane sums for each of Z planes, then
+// the kNG global sums.
+__host__ __device__ inline int ens_record(int Z) { return kNQ * Z + kNG; }
+
+// LDS of the step kernel: each wave's private per-plane sums.
+__host__ __device__ inline size_t ens_lds_bytes(int zmax) { return sizeof(double) * kEnsWaves * kNQ * (size_t)zmax; }
 
 // One member as the kernels see it.
 struct EnsMember {
     StepArgs a;      // buffers as at even steps of a launch sequence (src = state);
-                     // part/gpart/nblk: the member's tile partials and tile count
+                     // part: the member's tile records [ntiles][ens_record(Z)], nblk: ntiles
     double* planes;  // [kNQ*Z + kNG] reduced sums (the finisher's input)
-    int row0;        // first row of this member in the reduction's row space
-    int nrows;       // kNQ*Z + kNG
+    int chunk0;      // first 64-row chunk of this member in the reduction's chunk space
+    int nchunks;     // ceil(ens_record(Z) / 64)
 };
 
 struct EnsTile {
     int m;     // member
     int col0;  // first column
-    int ncol;  // columns (kEnsTileCols, fewer in a member's last tile)
-    int lt;    // tile index within the member (partial slot)
+    int ncol;  // columns
+    int lt;    // tile index within the member (record slot)
 };
 
 // Wave-uniform load through the constant address space (scalar loads) of
@@ -58,20 +61,29 @@ __device__ __forceinline__ S load_uniform(const S* p)
     return out;
 }
 
-// The fused step (step_column) over every tile of every member.  phase
-// selects the buffer direction: even steps read a.src, odd steps a.dst.
+// Orders the wave's LDS accesses across lanes (the zeroing and record
+// loops touch other lanes' plane sums): a wavefront-scope fence that the
+// compiler does not move memory operations across.
+__device__ __forceinline__ void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+// The fused step (step_column) over every tile of every member, one wave
+// per tile.  phase selects the buffer direction: even steps read a.src,
+// odd steps a.dst.
 template <typename T>
 static __global__ __launch_bounds__(kWave* kEnsWaves) void ens_step_kernel(const EnsMember* __restrict__ mem,
-                                                                   const EnsTile* __restrict__ tiles, int ntiles,
-                                                                   int phase)
+                                                                          const EnsTile* __restrict__ tiles,
+                                                                          int ntiles, int zmax, int phase)
 {
-    extern __shared__ double lds[];  // step_lds_bytes(max Z, kEnsWaves, true)
+    extern __shared__ double lds[];  // ens_lds_bytes(zmax)
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    constexpr int W = kEnsWaves;
-    double* red = lds;
-    double* planes = lds + (size_t)W * kNG * kWave;
-    for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    double* mine = lds + (size_t)wave * kNQ * zmax;
+    const int nw = gridDim.x * kEnsWaves;
+    for (int t = blockIdx.x * kEnsWaves + wave; t < ntiles; t += nw) {
         const EnsTile tl = load_uniform(tiles + t);
         StepArgs A = load_uniform(&mem[tl.m].a);
         if (phase) {
@@ -80,49 +92,78 @@ static __global__ __launch_bounds__(kWave* kEnsWaves) void ens_step_kernel(const
             A.dst = s;
         }
         const int Z = A.g.Z;
-        double* mine = planes + (size_t)wave * kNQ * Z;
-        double* shl = planes + (size_t)W * kNQ * Z;
         for (int i = lane; i < kNQ * Z; i += kWave) mine[i] = 0.0;
-        for (int i = threadIdx.x; i < 3 * Z; i += blockDim.x) shl[i] = A.shift[i];
-        __syncthreads();
+        wave_lds_sync();
         StepAcc acc;
         const double G = *A.G;
         const int cend = tl.col0 + tl.ncol;
         if (A.bands) {
-            for (int c = tl.col0 + wave; c < cend; c += W)
-                step_column<T, false, true, true>(A, c, lane, G, shl, mine, acc);
+            for (int c = tl.col0; c < cend; c++) step_column<T, false, true, true>(A, c, lane, G, A.shift, mine, acc);
         } else {
-            for (int c = tl.col0 + wave; c < cend; c += W)
-                step_column<T, false, true, false>(A, c, lane, G, shl, mine, acc);
+            for (int c = tl.col0; c < cend; c++) step_column<T, false, true, false>(A, c, lane, G, A.shift, mine, acc);
         }
-        step_block_reduce<true>(red, planes, acc, Z, W, wave, lane, A.part, A.gpart, A.nblk, tl.lt);
-        __syncthreads();  // the next tile reuses the LDS
+        wave_lds_sync();
+        double* rec = A.part + (long long)tl.lt * ens_record(Z);
+        for (int i = lane; i < kNQ * Z; i += kWave) rec[i] = mine[i];
+        double g[kNG] = {acc.jx, acc.mass, acc.mom, acc.sq};
+#pragma unroll
+        for (int q = 0; q < kNG; q++) {
+            double s = g[q];
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+            g[q] = s;
+        }
+        if (lane < kNG) {
+            double v = g[0];
+#pragma unroll
+            for (int q = 1; q < kNG; q++)
+                if (lane == q) v = g[q];
+            rec[kNQ * Z + lane] = v;
+        }
+        wave_lds_sync();  // the next tile zeroes the slice again
     }
 }
 
-// Fixed-order reduction of every member's tile partials; block b is row
-// b - row0 of the member whose rows contain b (plane rows kNQ*Z, then
-// kNG global rows).  Same arithmetic as reduce_rows_kernel over nblk tiles.
-static __global__ __launch_bounds__(256) void ens_reduce_kernel(const EnsMember* __restrict__ mem, int nmem)
+constexpr int kEnsReduceWaves = 16;  // waves per block of the record reduction
+
+// Fixed-order reduction of every member's tile records: block b covers 64
+// consecutive record rows (one per lane) of the member whose chunks
+// contain b.  Wave w sums the records t = w + 16 (4k + j) into accumulator
+// j in increasing k, the four accumulators are added pairwise, then the 16
+// wave sums in a fixed tree.  Reads are 512-byte record segments.
+static __global__ __launch_bounds__(kWave* kEnsReduceWaves) void ens_reduce_kernel(const EnsMember* __restrict__ mem,
+                                                                                  int nmem)
 {
-    __shared__ double sh[256];
+    __shared__ double sh[kEnsReduceWaves][kWave];
     const int b = blockIdx.x;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
     int m = 0;
-    while (m + 1 < nmem && load_uniform(&mem[m + 1].row0) <= b) m++;
-    const int r = b - load_uniform(&mem[m].row0);
+    while (m + 1 < nmem && load_uniform(&mem[m + 1].chunk0) <= b) m++;
     const int Z = load_uniform(&mem[m].a.g.Z);
     const int nt = load_uniform(&mem[m].a.nblk);
-    const double* in = r < kNQ * Z ? load_uniform(&mem[m].a.part) + (long long)r * nt
-                                   : load_uniform(&mem[m].a.gpart) + (long long)(r - kNQ * Z) * nt;
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nt; i += 256) s += in[i];
-    sh[threadIdx.x] = s;
+    const int nrec = ens_record(Z);
+    const int r = (b - load_uniform(&mem[m].chunk0)) * kWave + lane;
+    const double* part = load_uniform(&mem[m].a.part);
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    if (r < nrec) {
+        constexpr int S = kEnsReduceWaves;
+        int t = wave;
+        for (; t + 3 * S < nt; t += 4 * S) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) s[j] += part[(long long)(t + j * S) * nrec + r];
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            if (t + j * S < nt) s[j] += part[(long long)(t + j * S) * nrec + r];
+    }
+    sh[wave][lane] = (s[0] + s[1]) + (s[2] + s[3]);
     __syncthreads();
-    for (int w = 128; w >= 1; w >>= 1) {
-        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    for (int w = kEnsReduceWaves / 2; w >= 1; w >>= 1) {
+        if (wave < w) sh[wave][lane] += sh[wave + w][lane];
         __syncthreads();
     }
-    if (threadIdx.x == 0) load_uniform(&mem[m].planes)[r] = sh[0];
+    if (wave == 0 && r < nrec) load_uniform(&mem[m].planes)[r] = sh[0][lane];
 }
 
 // Finisher of every member (block m = member m): log row into slot

@@ -361,25 +361,10 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
             slow = (maxhb > 0 && z0 - 1 < maxhb) || (maxht > 0 && z1 >= Z - maxht);
             solid_own = band_solid(bw[1][1], z, Z);
         }
-        if (BANDS && slow && !FIRST) {
-            // each lane selects its source before the load: halfway
-            // bounce-back from its own column when the upstream node is
-            // solid (band or no-slip wall row), the pull otherwise
-            const char* s8 = (const char*)src;
-            static_for<0, Q>([&](auto ac) {
-                constexpr int a = decltype(ac)::value;
-                const int zs = z - EZ[a];
-                const bool solid_up =
-                    a != 0 && (zs < 0 || zs >= Z || band_solid(bw[EX[a] + 1][EY[a] + 1], zs, Z));
-                const long long e = (long long)a * Z + colbase<EX[a], EY[a]>(c) - EZ[a];
-                const uint32_t off = EZ[a] == 1 ? zoff_up : (EZ[a] == -1 ? zoff_dn : zoff);
-                const char* pn = s8 + e * (long long)sizeof(T) + off;
-                const char* pb = s8 + (c.cb + (long long)OPP[a] * Z) * (long long)sizeof(T) + zoff;
-                v[a] = *(const T*)(solid_up ? pb : pn);
-            });
-        } else {
-            gather_wave<T, FIRST>(src, c, zoff, zoff_up, zoff_dn, bot_lane, top_lane, z, ww, v);
-        }
+        // a node next to a solid band pulls, like any other, from its
+        // neighbour: the solid node's slot holds the bounced-back value
+        // (stored there by this node at the previous step, below)
+        gather_wave<T, FIRST>(src, c, zoff, zoff_up, zoff_dn, bot_lane, top_lane, z, ww, v);
 
         double mass_node, jx_raw, g11, g12, g13, g14;
         double gx[Q];  // post-collision values of the e_x != 0 directions (budget)
@@ -426,29 +411,50 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
         g13 = gx[13];
         g14 = gx[14];
         double mom_node = jx_raw + G;
-        if (BANDS && slow) {
-            // every push of an e_x != 0 population into a solid node (band
-            // or no-slip wall row) bounces back and reverses e_x
+        // bounce-back reverses e_x of the populations pushed into a
+        // no-slip wall node from a no-slip wall column (xcomputations.c:370)
+        if (bot_lane)
+            mom_node = mom_node - (((ww >> 10) & 1u) ? 2.0 * g12 : 0.0) + (((ww >> 11) & 1u) ? 2.0 * g13 : 0.0);
+        if (top_lane)
+            mom_node = mom_node - (((ww >> 12) & 1u) ? 2.0 * g11 : 0.0) + (((ww >> 13) & 1u) ? 2.0 * g14 : 0.0);
+        if (BANDS && slow && !solid_own) {
+            // and so does every push of an e_x != 0 population into a band
             static_for<1, Q>([&](auto ac) {
                 constexpr int a = decltype(ac)::value;
                 if constexpr (EX[a] != 0) {
                     const int zt = z + EZ[a];
-                    const bool solid_t = zt < 0 || zt >= Z || band_solid(bw[1 - EX[a]][1 - EY[a]], zt, Z);
-                    if (solid_t) mom_node -= 2.0 * (double)EX[a] * gx[a];
+                    if (zt >= 0 && zt < Z && band_solid(bw[1 - EX[a]][1 - EY[a]], zt, Z))
+                        mom_node -= 2.0 * (double)EX[a] * gx[a];
                 }
             });
-        } else {
-            // bounce-back reverses e_x of the populations pushed into a
-            // no-slip wall node from a no-slip wall column (xcomputations.c:370)
-            if (bot_lane)
-                mom_node = mom_node - (((ww >> 10) & 1u) ? 2.0 * g12 : 0.0) + (((ww >> 11) & 1u) ? 2.0 * g13 : 0.0);
-            if (top_lane)
-                mom_node = mom_node - (((ww >> 12) & 1u) ? 2.0 * g11 : 0.0) + (((ww >> 13) & 1u) ? 2.0 * g14 : 0.0);
         }
         if (!solid_own) {
             if constexpr (STATS) acc.jx += jx_raw;
             acc.mass += mass_node;
             acc.mom += mom_node;
+        }
+        if (BANDS && slow) {
+            // Halfway bounce-back at band surfaces, on the write side: the
+            // population a fluid node x pushes toward a solid node t = x + e_a
+            // returns to x as opp(a) at the next step, so x stores it in t's
+            // slot of direction opp(a), the slot x pulls from.  Every other
+            // slot of t is stored by t itself (values no fluid node reads):
+            // each slot is written once per step and every cache line of
+            // the destination is written whole.
+            static_for<0, Q>([&](auto ac) {
+                constexpr int a = decltype(ac)::value;
+                const int zt = z + EZ[a];
+                const bool inside = zt >= 0 && zt < Z;
+                // x + e_a is not a fluid node (a band node or a wall row)
+                const bool nb_solid = a == 0 ? solid_own : (!inside || band_solid(bw[1 - EX[a]][1 - EY[a]], zt, Z));
+                if (!solid_own || nb_solid)
+                    *(T*)(d8 + (c.cb + (long long)a * Z) * (long long)sizeof(T) + zoff) = v[a];
+                if (a != 0 && !solid_own && nb_solid && inside) {
+                    const long long e = colbase<-EX[a], -EY[a]>(c) + (long long)OPP[a] * Z + EZ[a];
+                    *(T*)(d8 + e * (long long)sizeof(T) + zoff) = v[a];
+                }
+            });
+        } else {
             static_for<0, Q>([&](auto ac) {
                 constexpr int a = decltype(ac)::value;
                 const long long e = c.cb + (long long)a * Z;

@@ -28,7 +28,7 @@ EXPORTS = (
     "dnsgpu_statistics", "dnsgpu_fields", "dnsgpu_time_steps", "dnsgpu_device_bytes",
     "dnsgpu_autotune", "dnsgpu_tavg_reset", "dnsgpu_tavg_sample", "dnsgpu_tavg_get",
     "dnsgpu_footprint",
-    "dnsens_create", "dnsens_destroy", "dnsens_member_bytes", "dnsens_add", "dnsens_remove",
+    "dnsens_create", "dnsens_destroy", "dnsens_member_bytes", "dnsens_footprint", "dnsens_add", "dnsens_remove",
     "dnsens_count", "dnsens_bytes_used", "dnsens_budget", "dnsens_step", "dnsens_info", "dnsens_rows",
     "dnsens_autotune", "dnsens_time_steps", "dnsens_grid",
 )
@@ -134,6 +134,8 @@ def load():
         L.dnsens_destroy.restype = None
         L.dnsens_member_bytes.argtypes = [vp]
         L.dnsens_member_bytes.restype = ctypes.c_size_t
+        L.dnsens_footprint.argtypes = [ctypes.POINTER(DnsCase)]
+        L.dnsens_footprint.restype = ctypes.c_size_t
         L.dnsens_add.argtypes = [vp, vp]
         L.dnsens_remove.argtypes = [vp, vp]
         L.dnsens_count.argtypes = [vp]
@@ -317,6 +319,12 @@ def footprint(xdv, ydv, zdv, precision=FP64):
     """Device bytes a handle of this grid takes (0: refused)."""
     c = DnsCase(xdv, ydv, zdv, 100.0, 0.1, 0.1, 0, CONST_Q, 0.0, precision, 0.0)
     return load().dnsgpu_footprint(ctypes.byref(c))
+
+
+def member_footprint(xdv, ydv, zdv, precision=FP64):
+    """Device bytes a case of this grid is charged as an ensemble member."""
+    c = DnsCase(xdv, ydv, zdv, 100.0, 0.1, 0.1, 0, CONST_Q, 0.0, precision, 0.0)
+    return load().dnsens_footprint(ctypes.byref(c))
 
 
 def _rows_dict(rows, n):

@@ -60,6 +60,7 @@ def load():
         L.dnsserve_case.restype = None
         L.dnsserve_sweep.argtypes = [ctypes.POINTER(Request), ctypes.c_int, ctypes.POINTER(Config),
                                      ctypes.POINTER(Result), ctypes.POINTER(Summary)]
+        L.dnsserve_open.argtypes = [ctypes.POINTER(Request), ctypes.POINTER(Config), ctypes.POINTER(ctypes.c_void_p)]
         L.dnsserve_sequential.argtypes = [ctypes.POINTER(Request), ctypes.c_int, ctypes.POINTER(Config), ctypes.c_int,
                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double)]
         _lib = L
@@ -114,3 +115,24 @@ def sequential(reqs, cfg, precision, steps):
     secs = (ctypes.c_double * n)()
     dnsgpu._chk(load().dnsserve_sequential(reqs, n, ctypes.byref(cfg), precision, st, secs))
     return list(secs)
+
+
+class Handle:
+    """A handle created, textured and initialised for a request
+    (dnsserve_open); usable as an ensemble member."""
+
+    def __init__(self, req, cfg):
+        self.h = ctypes.c_void_p()
+        dnsgpu._chk(load().dnsserve_open(ctypes.byref(req), ctypes.byref(cfg), ctypes.byref(self.h)))
+        self.X, self.Y, self.Z = req.xdv, req.ydv, req.zdv
+
+    def close(self):
+        if self.h:
+            dnsgpu.load().dnsgpu_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -250,7 +250,7 @@ int dnsserve_sweep(const dnsserve_request *req, int n, const dnsserve_config *cf
             dns_case c;
             dns_init init;
             dnsserve_case(req + i, cfg, &c, &init);
-            const size_t fp = dnsgpu_footprint(&c);
+            const size_t fp = dnsens_footprint(&c);
             if (fp == 0) { /* a case the library refuses */
                 res[i].status = -2;
                 res[i].finished = now_s() - t0;
@@ -264,11 +264,6 @@ int dnsserve_sweep(const dnsserve_request *req, int n, const dnsserve_config *cf
             if (rc == DNSGPU_ENOMEM) break;
             if (rc) goto out;
             rc = dnsens_add(e, h);
-            if (rc == DNSGPU_ENOMEM) { /* tile partials beyond the footprint */
-                dnsgpu_destroy(h);
-                rc = 0;
-                break;
-            }
             if (rc) {
                 dnsgpu_destroy(h);
                 goto out;

@@ -2,7 +2,7 @@
  * print one JSON object per solve and a summary line.
  *
  *   dnsserve [-n solves] [-s seed] [-g mean_gap_s] [-b] [-p fp32|fp64]
- *            [-c cap] [-w window] [-t transient] [-q seq_precision]
+ *            [-c cap] [-w window] [-t transient] [-k check] [-q seq_precision]
  *
  *   -b   burst: every request arrives at t = 0 (no arrival times)
  *   -q   also run the same solves one after another on the single-case
@@ -26,7 +26,7 @@ int main(int argc, char **argv)
     int n = 64, opt, seq = -1;
     unsigned long long seed = 5;
     double gap = 2.0;
-    while ((opt = getopt(argc, argv, "n:s:g:bp:c:w:t:q:")) != -1) {
+    while ((opt = getopt(argc, argv, "n:s:g:bp:c:w:t:k:q:")) != -1) {
         switch (opt) {
         case 'n': n = atoi(optarg); break;
         case 's': seed = strtoull(optarg, NULL, 10); break;
@@ -36,14 +36,18 @@ int main(int argc, char **argv)
         case 'c': cfg.cap = atoi(optarg); break;
         case 'w': cfg.window = atoi(optarg); break;
         case 't': cfg.transient = atoi(optarg); break;
+        case 'k': cfg.check = atoi(optarg); break;
         case 'q': seq = prec_of(optarg); break;
         default:
             fprintf(stderr, "usage: %s [-n solves] [-s seed] [-g gap] [-b] [-p fp32|fp64] [-c cap] [-w window] "
-                            "[-t transient] [-q fp64|fp32]\n", argv[0]);
+                            "[-t transient] [-k check] [-q fp64|fp32]\n", argv[0]);
             return 2;
         }
     }
-    if (n < 1) return 2;
+    if (n < 1 || cfg.check < 1 || cfg.window % cfg.check) {
+        fprintf(stderr, "dnsserve: need n >= 1, check >= 1 and window a multiple of check\n");
+        return 2;
+    }
     dnsserve_request *req = calloc((size_t)n, sizeof *req);
     dnsserve_result *res = calloc((size_t)n, sizeof *res);
     if (!req || !res) return 1;

@@ -122,7 +122,9 @@ int dnsgpu_set_walls(dnsgpu *h, const int8_t *bottom, const int8_t *top);
 /* Solid bands inside the fluid rows (riblets, posts with side walls): per
  * column the number of solid fluid rows above the bottom wall and below the
  * top wall, [xdv][ydv] each, NULL = 0.  No reference counterpart (the
- * reference only has solids in the wall rows); rule in the step kernel. */
+ * reference only has solids in the wall rows); rule in the step kernel.
+ * Set them before the first step: right after create, an init or a load
+ * (DNSGPU_EARG otherwise). */
 int dnsgpu_set_solid_bands(dnsgpu *h, const int16_t *bottom, const int16_t *top);
 
 /* Rest start, init.c:3-20 (the '-s' branch, dns-const-q-xycut.c:385-398). */
@@ -247,11 +249,13 @@ typedef struct dnsens dnsens;
 int dnsens_create(int precision, size_t hbm_budget, int flags, dnsens **out);
 void dnsens_destroy(dnsens *e);
 
-/* Device bytes a handle is charged as a member (its own plus any tile
- * partials the ensemble adds for it). */
+/* Device bytes a handle is charged as a member (its own plus the tile
+ * records of its partial sums, about 2-4 bytes per node). */
 size_t dnsens_member_bytes(const dnsgpu *h);
+/* The same for a case not yet created (0 when the case is refused). */
+size_t dnsens_footprint(const dns_case *c);
 
-/* Admit handle h (initialised or loaded, same precision, zdv <= 377).
+/* Admit handle h (initialised or loaded, same precision).
  * Returns DNSGPU_ENOMEM, with nothing changed, when its bytes do not fit
  * the budget.  A state fresh from an init or a load takes its first step
  * here (one logged step). */

@@ -153,8 +153,10 @@ def test_admission_by_footprint_and_member_rules():
     print("case: admission", flush=True)
     chs = [make((16, 8, 10), "smooth", precision=dnsgpu.FP32, seed=s) for s in (1, 2, 3)]
     need = dnsgpu.Ensemble.member_bytes(chs[0])
-    assert need == chs[0].device_bytes() > 0
+    # the handle plus its tile records: 128 columns in tiles of 8, 4*10+4 doubles each
+    assert need == chs[0].device_bytes() + 16 * 44 * 8
     assert dnsgpu.footprint(16, 8, 10, dnsgpu.FP32) == chs[0].device_bytes()
+    assert dnsgpu.member_footprint(16, 8, 10, dnsgpu.FP32) == need
     ens = dnsgpu.Ensemble(precision=dnsgpu.FP32, budget=2 * need + need // 2)
     ens.add(chs[0])
     ens.add(chs[1])

@@ -55,7 +55,7 @@ def test_admission_waits_for_memory():
     # three solves of one grid: a budget of 1.5 footprints holds one at a time
     items = [dict(SMALL[0], seed=11), dict(SMALL[0], texture=dnsserve.RIBLETS, period=8, param=4.0, seed=12),
              dict(SMALL[0], param=0.25, seed=13)]
-    need = dnsgpu.footprint(32, 16, 24, dnsgpu.FP32)
+    need = dnsgpu.member_footprint(32, 16, 24, dnsgpu.FP32)
     res, summ = dnsserve.sweep(dnsserve.requests(items), cfg(budget=need + need // 2))
     assert summ["max_resident"] == 1 and summ["completed"] == 3
     # served one at a time, in arrival order
