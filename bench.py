@@ -109,13 +109,13 @@ def run_precision(wname, prec, steps, warmup):
     }
 
 
-def cpu_baseline(wname, budget_s=12.0):
+def cpu_baseline(wname, budget_s=12.0, w=None):
     """The oracle (CPU restatement of the reference loop, one core) on a
     bounded sample of the workload: the same y/z extents with 4 x-planes."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import numpy as np
     import pyoracle
-    w = WORKLOADS[wname]
+    w = w or WORKLOADS[wname]
     omega, F, ub, nu = case_params(w)
     X, Y, Z = 4, w["Y"], w["Z"]
     o = pyoracle.Oracle(X, Y, Z, omega, ub, 0.1, forcing=1, F=F)
@@ -144,20 +144,200 @@ def measured_traffic(wname):
         return json.load(fh)
 
 
+SWEEP = "texture_sweep_64"
+
+# Drag reduction of a textured channel against the smooth one at the same
+# constant body force, FP64 and FP32-shifted from the same seeded field:
+# time averages over [avg_from, steps).
+DRAG = {
+    "drag_retau180_grooves_768x384x192": dict(base="retau180_768x384x192", texture=("grooves", 32, 16),
+                                              steps=20000, avg_from=10000),
+    "drag_retau395_riblets_1536x768x384": dict(base="retau395_1536x768x384", texture=("riblets", 16, 8),
+                                               steps=4000, avg_from=2000),
+}
+
+
+def stripes(X, Y, period, slip):
+    """Streamwise-invariant slip stripes on a wall (code -2 slip, -1 no-slip)."""
+    import numpy as np
+    row = np.where(np.arange(Y) % period < slip, -2, -1).astype(np.int8)
+    return np.ascontiguousarray(np.broadcast_to(row[None, :], (X, Y)))
+
+
+def run_drag(name, args):
+    import numpy as np
+    d = DRAG[name]
+    w = WORKLOADS[d["base"]]
+    omega, F, ub, nu = case_params(w)
+    X, Y, Z = w["X"], w["Y"], w["Z"]
+    steps = args.dr_steps or d["steps"]
+    avg_from = min(args.dr_avg_from or d["avg_from"], steps - 500)
+    runs = {}
+    for prec in (dnsgpu.FP64, dnsgpu.FP32):
+        for textured in (False, True):
+            ch = dnsgpu.Channel(X, Y, Z, Rebs=ub * Z / nu, ub=ub, CFL=0.1, TSTR=0, forcing=dnsgpu.CONST_F,
+                                force=F, precision=prec, omega=omega)
+            try:
+                kind, a, b = d["texture"]
+                if textured and kind == "grooves":
+                    m = stripes(X, Y, a, b)
+                    ch.set_walls(m, m)
+                elif textured:
+                    ch.set_solid_bands(*([riblet_bands(X, Y, a, b)] * 2))
+                ch.init_field(dnsgpu.INIT_TURBULENT, uc=w["utau"], amp=0.1, seed=2, nmodes=4)
+                ch.autotune(trial_steps=3)
+                t0 = time.perf_counter()
+                ch.step(avg_from, cntr0=0, logs=False)
+                ch.tavg_reset()
+                done = avg_from
+                while done < steps:
+                    k = min(500, steps - done)
+                    ch.step(k, cntr0=done, logs=True)
+                    ch.tavg_sample()
+                    done += k
+                wall = time.perf_counter() - t0
+                tv = ch.tavg(profiles=True)
+            finally:
+                ch.close()
+            runs[(prec, textured)] = (tv, wall)
+            print(f"# {'fp64' if prec == dnsgpu.FP64 else 'fp32'} {'textured' if textured else 'smooth'}: "
+                  f"Ub={tv['Ub']:.6e} {wall:.1f} s", file=sys.stderr, flush=True)
+
+    def dr(prec):
+        return dnsgpu.drag_reduction(runs[(prec, False)][0], runs[(prec, True)][0], dnsgpu.CONST_F)["DR_Ub"]
+
+    t64, t32 = runs[(dnsgpu.FP64, True)][0], runs[(dnsgpu.FP32, True)][0]
+    u64, u32 = t64["profiles"][0], t32["profiles"][0]
+    prof_err = float(np.abs(u32 - u64).max() / np.abs(u64).max())
+    ut64, ut32 = np.sqrt(abs(t64["stress"])), np.sqrt(abs(t32["stress"]))
+    nodes = X * Y * Z
+    out = {
+        "metric": "MLUPS + % HBM roofline (FP32 vs FP64 D3Q19); ensemble solves/h + p99 latency",
+        "value": round(nodes * steps / runs[(dnsgpu.FP64, True)][1] / 1e6, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1, "steps": steps, "warmup": 0,
+        "ms_per_step": round(runs[(dnsgpu.FP64, True)][1] / steps * 1e3, 4),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": name, "grid_xdv_ydv_zdv": [X, Y, Z], "texture": "%s %d %d" % d["texture"],
+                   "forcing": "constant body force", "averaging_window": [avg_from, steps],
+                   "parallelism": "single GPU"},
+        "drag_reduction_percent": {"fp64": round(dr(dnsgpu.FP64), 4), "fp32_shifted": round(dr(dnsgpu.FP32), 4),
+                                   "error_points": round(abs(dr(dnsgpu.FP32) - dr(dnsgpu.FP64)), 4),
+                                   "definition": "1 - (Ub_smooth / Ub_textured)^2 at constant force"},
+        "mean_profile_max_rel_error": prof_err,
+        "u_tau_wall_stress": {"fp64": ut64, "fp32_shifted": ut32, "rel_error": abs(ut32 - ut64) / ut64},
+        "slip_velocity": {"fp64": 0.5 * (t64["u_slip_bottom"] + t64["u_slip_top"]),
+                          "fp32_shifted": 0.5 * (t32["u_slip_bottom"] + t32["u_slip_top"])},
+        "Ub": {f"{'fp64' if p == dnsgpu.FP64 else 'fp32'}_{'textured' if t else 'smooth'}": runs[(p, t)][0]["Ub"]
+               for (p, t) in runs},
+        "wall_s": {f"{'fp64' if p == dnsgpu.FP64 else 'fp32'}_{'textured' if t else 'smooth'}": round(runs[(p, t)][1], 2)
+                   for (p, t) in runs},
+    }
+    sv = out["slip_velocity"]
+    sv["rel_error"] = abs(sv["fp32_shifted"] - sv["fp64"]) / abs(sv["fp64"]) if sv["fp64"] else None
+    return out
+
+
+def run_sweep(args):
+    """The serving ensemble texture sweep: 64 seeded FP32-shifted solves
+    (driver/dnsserve.c: grids, Re_tau, grooves or riblets, Poisson arrivals
+    of mean gap 2 s) served through one ensemble until the running u_tau
+    settles (0.5% over 2000 steps after a 2000-step transient, cap 10000);
+    then the same solves, for the same step counts, one after another on
+    the plain FP64 single-case path."""
+    import dnsserve
+    reqs = dnsserve.generate(64, seed=5, mean_gap=2.0)
+    cfg = dnsserve.defaults(realtime=0 if args.burst else 1, verbose=1)
+    res, summ = dnsserve.sweep(reqs, cfg)
+    print(f"# sweep done: {summ['wall']:.1f} s, {summ['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+    steps = [r["steps"] if r["status"] >= 0 else 0 for r in res]
+    seq = None
+    if not args.no_sequential:
+        seq = sum(dnsserve.sequential(reqs, cfg, dnsgpu.FP64, steps))
+        print(f"# sequential fp64 done: {seq:.1f} s", file=sys.stderr, flush=True)
+    # the dominant kernel: the multi-case step with all 64 solves resident
+    hs = [dnsserve.Handle(r, cfg) for r in reqs]
+    ens = dnsgpu.Ensemble(precision=dnsgpu.FP32)
+    for h in hs:
+        ens.add(h)
+    blocks, _ = ens.autotune(3)
+    ens.step(2)
+    n = 10
+    tk, tw = ens.time_kernels(n)
+    ens.close()
+    for h in hs:
+        h.close()
+    nodes = sum(r.xdv * r.ydv * r.zdv for r in reqs)
+    achieved = nodes * 152 / (tk / n / 1e3) / 1e9
+    status = {str(k): sum(1 for r in res if r["status"] == k) for k in (1, 0, -1, -2)}
+    out = {
+        "metric": "MLUPS + % HBM roofline (FP32 vs FP64 D3Q19); ensemble solves/h + p99 latency",
+        "value": round(summ["solves_per_hour"], 1),
+        "unit": "solves/h",
+        "n_gpus": 1,
+        "steps": int(sum(steps)),
+        "warmup": 0,
+        "ms_per_step": round(tw / n, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": SWEEP, "solves": 64, "seed": 5,
+                   "arrivals": "all at t=0" if args.burst else "Poisson, mean gap 2 s",
+                   "grids": "{128,256,512} x {64,128,256} x {64,96,128} (xdv x ydv x zdv)",
+                   "re_tau": "U[90,180]", "textures": "grooves (period 8-64, gas fraction 0.3-0.9) or riblets "
+                   "(spacing 8-32, height 4-12)", "convergence": "running u_tau within 0.5% over 2000 steps "
+                   "after a 2000-step transient, cap 10000", "parallelism": "single GPU, one ensemble"},
+        "p50_s": round(summ["p50"], 3),
+        "p99_s": round(summ["p99"], 3),
+        "mean_time_to_result_s": round(summ["mean_ttr"], 3),
+        "wall_s": round(summ["wall"], 3),
+        "busy_s": round(summ["busy"], 3),
+        "aggregate_mlups": round(summ["mlups"], 1),
+        "aggregate_mlups_busy": round(summ["mlups_busy"], 1),
+        "max_resident": summ["max_resident"],
+        "peak_device_bytes": summ["peak_bytes"],
+        "status_counts": status,
+        "sequential_fp64_s": None if seq is None else round(seq, 3),
+        "speedup_vs_sequential_fp64": None if seq is None else round(seq / summ["wall"], 3),
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "kernel": "ens_step_kernel<float>", "kernel_ms": round(tk / n, 4), "bytes_per_node": 152,
+                     "nodes": nodes, "grid_blocks": blocks},
+        "solves": res,
+    }
+    if not args.no_cpu:
+        r0 = reqs[0]
+        w = dict(X=r0.xdv, Y=r0.ydv, Z=r0.zdv, retau=None, utau=None)
+        out["cpu_baseline"] = cpu_baseline(SWEEP, w=w)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--workload", default="retau180_768x384x192", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="retau180_768x384x192", choices=sorted(WORKLOADS) + [SWEEP] + sorted(DRAG))
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fp64-only", action="store_true")
+    ap.add_argument("--burst", action="store_true", help=f"{SWEEP}: every request at t = 0")
+    ap.add_argument("--no-sequential", action="store_true", help=f"{SWEEP}: skip the sequential FP64 solves")
+    ap.add_argument("--dr-steps", type=int, default=0, help="drag workloads: steps per run")
+    ap.add_argument("--dr-avg-from", type=int, default=0, help="drag workloads: first averaged step")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs the whole channel on one GPU; multi-GPU runs are not part of it "
               "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
     dnsgpu.load()
+    if args.workload == SWEEP:
+        print(json.dumps(run_sweep(args)))
+        return
+    if args.workload in DRAG:
+        print(json.dumps(run_drag(args.workload, args)))
+        return
     r64 = run_precision(args.workload, dnsgpu.FP64, args.steps, args.warmup)
     print(f"# fp64 done: {r64['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
     r32 = None

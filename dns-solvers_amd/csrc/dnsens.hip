@@ -85,7 +85,7 @@ extern "C" int dnsens_create(int precision, size_t hbm_budget, int flags, dnsens
 {
     if (!out) return set_err(DNSGPU_EARG, "null argument");
     *out = nullptr;
-    if (precision != DNSGPU_FP64 && precision != DNSGPU_FP32)
+    if (precision != DNSGPU_FP64 && precision != DNSGPU_FP32 && precision != DNSGPU_FP16)
         return set_err(DNSGPU_EARG, "precision %d not built", precision);
     dnsens* e = new (std::nothrow) dnsens();
     if (!e) return set_err(DNSGPU_ENOMEM, "host allocation failed");
@@ -333,7 +333,7 @@ static int upload_tables(dnsens* e)
         e->zmax = zmax;
         e->lds = ens_lds_bytes(std::max(zmax, 1));
         int per = 0;
-        int rc = e->precision == DNSGPU_FP64 ? step_occupancy<double>(e->lds, &per) : step_occupancy<float>(e->lds, &per);
+        int rc = with_storage(e->precision, [&](auto t) { return step_occupancy<decltype(t)>(e->lds, &per); });
         if (rc) return rc;
         e->resident = std::max(1, per) * e->ncu;
         e->layout_changed = true;
@@ -377,11 +377,11 @@ static int enqueue_steps(dnsens* e, int k)
 static int run_chunk(dnsens* e, int k)
 {
     const bool use_graph = !(e->flags & DNSENS_NO_GRAPH);
-    if (!use_graph) return e->precision == DNSGPU_FP64 ? enqueue_steps<double>(e, k) : enqueue_steps<float>(e, k);
+    if (!use_graph) return with_storage(e->precision, [&](auto t) { return enqueue_steps<decltype(t)>(e, k); });
     if (e->layout_changed || e->graph_steps != k || !e->exec) {
         drop_graph(e);
         HIPCHK(hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-        int rc = e->precision == DNSGPU_FP64 ? enqueue_steps<double>(e, k) : enqueue_steps<float>(e, k);
+        int rc = with_storage(e->precision, [&](auto t) { return enqueue_steps<decltype(t)>(e, k); });
         hipGraph_t g = nullptr;
         hipError_t ce = hipStreamEndCapture(e->stream, &g);
         if (rc) {
@@ -529,8 +529,7 @@ static int tune_now(dnsens* e, int trials, int* best_blocks, double* best_ms)
     int bpc = e->blocks_per_cu;
     double ms = -1.0;
     if (!e->mem.empty()) {
-        int rc = e->precision == DNSGPU_FP64 ? tune_grid<double>(e, trials, &bpc, &ms)
-                                             : tune_grid<float>(e, trials, &bpc, &ms);
+        int rc = with_storage(e->precision, [&](auto t) { return tune_grid<decltype(t)>(e, trials, &bpc, &ms); });
         if (rc) return rc;
         g_ens_tune.push_back({e->precision, e->zmax, ilog2(std::max(1, e->ntiles)), bpc});
     }
@@ -570,6 +569,49 @@ extern "C" int dnsens_time_steps(dnsens* e, int nsteps, double* ms)
     float t = 0.f;
     HIPCHK(hipEventElapsedTime(&t, e->ev0, e->ev1));
     *ms = t;
+    return probe(e);
+}
+
+extern "C" int dnsens_time_kernels(dnsens* e, int nsteps, double* ms_step_kernels, double* ms_total)
+{
+    if (!e || nsteps < 1) return set_err(DNSGPU_EARG, "bad argument");
+    if (e->mem.empty()) return set_err(DNSGPU_EARG, "empty ensemble");
+    if (e->tables_stale) {
+        int rc = upload_tables(e);
+        if (rc) return rc;
+    }
+    const int nm = (int)e->mem.size();
+    std::vector<hipEvent_t> ev(2 * (size_t)nsteps);
+    for (auto& x : ev) HIPCHK(hipEventCreate(&x));
+    HIPCHK(hipEventRecord(e->ev0, e->stream));
+    for (int s = 0; s < nsteps; s++) {
+        HIPCHK(hipEventRecord(ev[2 * s], e->stream));
+        with_storage(e->precision, [&](auto t) { enqueue_step_only<decltype(t)>(e, s & 1); });
+        HIPCHK(hipEventRecord(ev[2 * s + 1], e->stream));
+        hipLaunchKernelGGL(ens_reduce_kernel, dim3(e->nchunks), dim3(kWave * kEnsReduceWaves), 0, e->stream, e->d_mem,
+                           nm);
+        hipLaunchKernelGGL(ens_finalize_kernel, dim3(nm), dim3(1024), 0, e->stream, e->d_fin, s);
+    }
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipEventRecord(e->ev1, e->stream));
+    HIPCHK(hipEventSynchronize(e->ev1));
+    double tk = 0.0;
+    for (int s = 0; s < nsteps; s++) {
+        float ms = 0.f;
+        HIPCHK(hipEventElapsedTime(&ms, ev[2 * s], ev[2 * s + 1]));
+        tk += ms;
+    }
+    float wall = 0.f;
+    HIPCHK(hipEventElapsedTime(&wall, e->ev0, e->ev1));
+    for (auto& x : ev) (void)hipEventDestroy(x);
+    for (auto& m : e->mem) {
+        if (nsteps & 1) m.h->cur = 1 - m.h->cur;
+        m.h->poststream = 0;
+    }
+    if (nsteps & 1) e->tables_stale = true;
+    e->last_chunk = nsteps;
+    if (ms_step_kernels) *ms_step_kernels = tk;
+    if (ms_total) *ms_total = wall;
     return probe(e);
 }
 

@@ -68,6 +68,7 @@ StepArgs step_args(dnsgpu* h, const void* src, void* dst, int nblk)
     A.mtop = h->mtop;
     A.wword = h->wword;
     A.bands = h->has_bands ? h->bands : nullptr;
+    A.bnbr = h->has_bands ? h->bnbr : nullptr;
     A.omega = h->omega;
     A.G = &h->st->G;
     A.shift = h->shift;
@@ -161,8 +162,7 @@ static int launch_step(dnsgpu* h, bool stats, LogRow* row, long long cntr)
 
 int do_step(dnsgpu* h, bool stats, LogRow* row, long long cntr)
 {
-    if (h->c.precision == DNSGPU_FP64) return launch_step<double>(h, stats, row, cntr);
-    return launch_step<float>(h, stats, row, cntr);
+    return with_storage(h->c.precision, [&](auto t) { return launch_step<decltype(t)>(h, stats, row, cntr); });
 }
 
 // Derive the force of the next step and the plane-mean shifts from the
@@ -187,8 +187,7 @@ static int launch_moments(dnsgpu* h)
 static int refresh_force(dnsgpu* h)
 {
     HIPCHK(hipMemsetAsync(h->shift, 0, sizeof(double) * 3 * h->g.Z, h->stream));
-    if (h->c.precision == DNSGPU_FP64) return launch_moments<double>(h);
-    return launch_moments<float>(h);
+    return with_storage(h->c.precision, [&](auto t) { return launch_moments<decltype(t)>(h); });
 }
 
 static void count_slip_columns(dnsgpu* h, const int8_t* bottom, const int8_t* top);
@@ -210,7 +209,7 @@ static int plan_handle(const dns_case* c, dnsgpu* h)
                        kWave * kMaxWaves);
     if ((long long)c->xdv * c->ydv * c->zdv >= (1ll << 31))
         return set_err(DNSGPU_EARG, "grid has more than 2^31 nodes");
-    if (c->precision != DNSGPU_FP64 && c->precision != DNSGPU_FP32)
+    if (c->precision != DNSGPU_FP64 && c->precision != DNSGPU_FP32 && c->precision != DNSGPU_FP16)
         return set_err(DNSGPU_EARG, "precision %d not built", c->precision);
     h->c = *c;
     Geom& g = h->g;
@@ -224,7 +223,7 @@ static int plan_handle(const dns_case* c, dnsgpu* h)
     g.total = (long long)g.ncols * Q * g.Z;
     h->block = kWave * g.nzc * g.cpb;
     h->step_w = step_waves(g.Z);
-    h->esize = c->precision == DNSGPU_FP64 ? 8 : 4;
+    h->esize = storage_bytes(c->precision);
     // omega and the constants exactly as dns-const-q-xycut.c:211-217,
     // wshearstr.c:24, encalc.c:20 evaluate them.
     const int zd = g.Z + 2;
@@ -242,8 +241,9 @@ static int plan_handle(const dns_case* c, dnsgpu* h)
     // optimum of the read+write stream (overridden by the autotuner or
     // DNSGPU_STEP_BLOCKS)
     int occ = 0;
-    int rc = c->precision == DNSGPU_FP64 ? occupancy_blocks(moments_kernel<double, false>, h->block, &occ)
-                                         : occupancy_blocks(moments_kernel<float, false>, h->block, &occ);
+    int rc = with_storage(c->precision, [&](auto t) {
+        return occupancy_blocks(moments_kernel<decltype(t), false>, h->block, &occ);
+    });
     if (rc) return rc;
     h->nblk = std::max(1, std::min(occ, g.ngroups));
     int dev = 0, ncu = 0;
@@ -255,12 +255,11 @@ static int plan_handle(const dns_case* c, dnsgpu* h)
     // never more blocks than can be resident at once (the grid loops)
     int per = 0;
     const size_t lds_stats = step_lds_bytes(g.Z, h->step_w, true);
-    if (c->precision == DNSGPU_FP64)
-        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, step_kernel<double, false, true, true>,
-                                                            kWave * h->step_w, lds_stats));
-    else
-        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, step_kernel<float, false, true, true>,
-                                                            kWave * h->step_w, lds_stats));
+    const hipError_t oe = with_storage(c->precision, [&](auto t) {
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, step_kernel<decltype(t), false, true, true>,
+                                                            kWave * h->step_w, lds_stats);
+    });
+    HIPCHK(oe);
     h->step_resident = std::max(1, per) * ncu;
     sb = std::min(sb, h->step_resident);
     const char* sbe = getenv("DNSGPU_STEP_BLOCKS");
@@ -290,6 +289,7 @@ static std::vector<DevAlloc> handle_allocs(dnsgpu* h)
         {(void**)&h->mtop, xy},
         {(void**)&h->wword, xy * 4},
         {(void**)&h->bands, xy * 4},
+        {(void**)&h->bnbr, xy * sizeof(BandNbr)},
         {(void**)&h->nf, sizeof(double) * g.Z},
         {(void**)&h->part, sizeof(double) * kNStat * g.Z * np},
         {(void**)&h->gpart, sizeof(double) * kNG * np},
@@ -362,7 +362,7 @@ extern "C" void dnsgpu_destroy(dnsgpu* h)
     if (!h) return;
     if (h->owner) (void)dnsens_remove(h->owner, h);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    void* ptrs[] = {h->buf[0], h->buf[1], h->mbot,  h->mtop,  h->wword, h->bands, h->nf,
+    void* ptrs[] = {h->buf[0], h->buf[1], h->mbot,  h->mtop,  h->wword, h->bands, h->bnbr, h->nf,
                     h->part,   h->gpart,  h->planes, h->shift, h->means, h->tprof, h->slip,
                     h->st,     h->ring,   h->stage};
     for (void* p : ptrs)
@@ -450,10 +450,7 @@ extern "C" int dnsgpu_autotune(dnsgpu* h, int trial_steps, int* best_blocks, dou
         // one untimed launch, then trial_steps timed ones; buffers are not flipped
         for (int r = 0; r <= trial_steps; r++) {
             if (r == 1) HIPCHK(hipEventRecord(h->ev0, h->stream));
-            if (h->c.precision == DNSGPU_FP64)
-                enqueue_step_kernel<double>(h, true);
-            else
-                enqueue_step_kernel<float>(h, true);
+            with_storage(h->c.precision, [&](auto t) { enqueue_step_kernel<decltype(t)>(h, true); });
             HIPCHK(hipGetLastError());
         }
         HIPCHK(hipEventRecord(h->ev1, h->stream));
@@ -547,6 +544,9 @@ extern "C" int dnsgpu_set_solid_bands(dnsgpu* h, const int16_t* bottom, const in
     double total = 0.0;
     for (int z = 0; z < Z; z++) total += nf[z];
     HIPCHK(hipMemcpyAsync(h->bands, bw.data(), xy * 4, hipMemcpyHostToDevice, h->stream));
+    hipLaunchKernelGGL(band_nbr_kernel, dim3((unsigned)((xy + 255) / 256)), dim3(256), 0, h->stream, h->bands, X, Y,
+                       h->bnbr);
+    HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(h->nf, nf.data(), sizeof(double) * Z, hipMemcpyHostToDevice, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
     h->has_bands = any ? 1 : 0;
@@ -573,10 +573,9 @@ static int launch_init(dnsgpu* h, int kind, const dns_init* spec)
     A.Bplus = 5.2;
     const long long n = (long long)h->g.X * h->g.Y * h->g.Z;
     int grid = (int)std::min<long long>((n + 255) / 256, 65536);
-    if (h->c.precision == DNSGPU_FP64)
-        hipLaunchKernelGGL(init_kernel<double>, dim3(grid), dim3(256), 0, h->stream, A);
-    else
-        hipLaunchKernelGGL(init_kernel<float>, dim3(grid), dim3(256), 0, h->stream, A);
+    with_storage(h->c.precision, [&](auto t) {
+        hipLaunchKernelGGL(init_kernel<decltype(t)>, dim3(grid), dim3(256), 0, h->stream, A);
+    });
     HIPCHK(hipGetLastError());
     h->cur = 0;
     h->poststream = 1;
@@ -631,10 +630,9 @@ extern "C" int dnsgpu_load_aos(dnsgpu* h, int x0, int nx, const double* ftemp, c
     A.f = (double*)((char*)h->stage + half);
     const long long n = (long long)nx * h->g.Y * h->g.Z;
     int grid = (int)std::min<long long>((n + 255) / 256, 65536);
-    if (h->c.precision == DNSGPU_FP64)
-        hipLaunchKernelGGL(load_aos_kernel<double>, dim3(grid), dim3(256), 0, h->stream, A);
-    else
-        hipLaunchKernelGGL(load_aos_kernel<float>, dim3(grid), dim3(256), 0, h->stream, A);
+    with_storage(h->c.precision, [&](auto t) {
+        hipLaunchKernelGGL(load_aos_kernel<decltype(t)>, dim3(grid), dim3(256), 0, h->stream, A);
+    });
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(h->stream));
     h->cur = 0;
@@ -676,10 +674,9 @@ extern "C" int dnsgpu_save_aos(dnsgpu* h, int x0, int nx, double* ftemp, double*
     A.f = (double*)((char*)h->stage + half);
     const long long n = (long long)nx * h->g.Y * (h->g.Z + 2);
     int grid = (int)std::min<long long>((n + 255) / 256, 65536);
-    if (h->c.precision == DNSGPU_FP64)
-        hipLaunchKernelGGL(save_aos_kernel<double>, dim3(grid), dim3(256), 0, h->stream, A);
-    else
-        hipLaunchKernelGGL(save_aos_kernel<float>, dim3(grid), dim3(256), 0, h->stream, A);
+    with_storage(h->c.precision, [&](auto t) {
+        hipLaunchKernelGGL(save_aos_kernel<decltype(t)>, dim3(grid), dim3(256), 0, h->stream, A);
+    });
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(ftemp, h->stage, half, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipMemcpyAsync(f, (char*)h->stage + half, half, hipMemcpyDeviceToHost, h->stream));
@@ -763,7 +760,7 @@ static int run_statistics(dnsgpu* h, std::vector<double>& m, std::vector<double>
     const double nxy = (double)(h->g.X * h->g.Y);
     s.assign(5 * (size_t)Z, 0.0);
     m.assign(4 * (size_t)Z, 0.0);
-    int rc = h->c.precision == DNSGPU_FP64 ? launch_stats_pass<double>(h, 1) : launch_stats_pass<float>(h, 1);
+    int rc = with_storage(h->c.precision, [&](auto t) { return launch_stats_pass<decltype(t)>(h, 1); });
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(s.data(), h->planes, sizeof(double) * 5 * Z, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -771,7 +768,7 @@ static int run_statistics(dnsgpu* h, std::vector<double>& m, std::vector<double>
     for (int q = 0; q < 4; q++)
         for (int z = 0; z < Z; z++) m[(size_t)q * Z + z] = s[(size_t)q * Z + z] / nxy;
     HIPCHK(hipMemcpyAsync(h->means, m.data(), sizeof(double) * 4 * Z, hipMemcpyHostToDevice, h->stream));
-    rc = h->c.precision == DNSGPU_FP64 ? launch_stats_pass<double>(h, 2) : launch_stats_pass<float>(h, 2);
+    rc = with_storage(h->c.precision, [&](auto t) { return launch_stats_pass<decltype(t)>(h, 2); });
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(s.data(), h->planes, sizeof(double) * 5 * Z, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -851,7 +848,7 @@ extern "C" int dnsgpu_tavg_sample(dnsgpu* h)
     hipLaunchKernelGGL(tavg_profile_kernel, dim3((Z + 255) / 256), dim3(256), 0, h->stream, h->means, h->planes, Z,
                        (double)h->g.X * h->g.Y, h->tprof, h->st);
     HIPCHK(hipGetLastError());
-    rc = h->c.precision == DNSGPU_FP64 ? launch_slip<double>(h) : launch_slip<float>(h);
+    rc = with_storage(h->c.precision, [&](auto t) { return launch_slip<decltype(t)>(h); });
     if (rc) return rc;
     std::vector<double> part(4 * (size_t)kSlipBlocks);
     HIPCHK(hipMemcpyAsync(part.data(), h->slip, sizeof(double) * part.size(), hipMemcpyDeviceToHost, h->stream));
@@ -935,10 +932,9 @@ extern "C" int dnsgpu_fields(dnsgpu* h, int x0, int nx, double* vel, double* den
     A.vel = h->stage;
     A.den = h->stage + nn * 3;
     int grid = (int)std::min<size_t>((nn + 255) / 256, 65536);
-    if (h->c.precision == DNSGPU_FP64)
-        hipLaunchKernelGGL(fields_kernel<double>, dim3(grid), dim3(256), 0, h->stream, A);
-    else
-        hipLaunchKernelGGL(fields_kernel<float>, dim3(grid), dim3(256), 0, h->stream, A);
+    with_storage(h->c.precision, [&](auto t) {
+        hipLaunchKernelGGL(fields_kernel<decltype(t)>, dim3(grid), dim3(256), 0, h->stream, A);
+    });
     HIPCHK(hipGetLastError());
     HIPCHK(hipMemcpyAsync(vel, A.vel, nn * 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipMemcpyAsync(den, A.den, nn * sizeof(double), hipMemcpyDeviceToHost, h->stream));
@@ -956,10 +952,7 @@ extern "C" int dnsgpu_time_steps(dnsgpu* h, int nsteps, int with_stats, double* 
     for (int s = 0; s < nsteps; s++) {
         HIPCHK(hipEventRecord(ev[2 * s], h->stream));
         const bool st = with_stats != 0;
-        if (h->c.precision == DNSGPU_FP64)
-            enqueue_step_kernel<double>(h, st);
-        else
-            enqueue_step_kernel<float>(h, st);
+        with_storage(h->c.precision, [&](auto t) { enqueue_step_kernel<decltype(t)>(h, st); });
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(ev[2 * s + 1], h->stream));
         int rc = reduce_partials(h, st, h->step_nblk);

@@ -36,6 +36,7 @@ struct dnsgpu {
     int8_t* mtop;
     uint32_t* wword;
     uint32_t* bands;  // [X*Y] solid band words hb | ht << 16
+    uint32_t* bnbr;   // [X*Y][16] band neighbourhood records
     double* nf;       // [Z] fluid nodes per plane
     double nfluid;    // fluid nodes in total
     int has_bands;
@@ -74,6 +75,22 @@ struct dnsgpu {
         if ((h)->owner)                                                                \
             return set_err(DNSGPU_EARG, "%s: the handle is an ensemble member", __func__); \
     } while (0)
+
+// Calls f with a value of the storage type of a precision (FP64: double,
+// FP32: float, FP16: _Float16) as a type tag.
+template <typename F>
+inline auto with_storage(int precision, F&& f)
+{
+    if (precision == DNSGPU_FP64) return f(double{});
+    if (precision == DNSGPU_FP16) return f(_Float16{});
+    return f(float{});
+}
+
+// Bytes per stored population of a precision.
+inline int storage_bytes(int precision)
+{
+    return precision == DNSGPU_FP64 ? 8 : (precision == DNSGPU_FP16 ? 2 : 4);
+}
 
 // Step arguments of the handle's current state (src) into dst.
 __attribute__((visibility("hidden"))) StepArgs step_args(dnsgpu* h, const void* src, void* dst, int nblk);

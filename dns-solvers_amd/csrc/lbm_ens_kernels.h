@@ -45,22 +45,6 @@ struct EnsTile {
     int lt;    // tile index within the member (record slot)
 };
 
-// Wave-uniform load through the constant address space (scalar loads) of
-// a table entry that no kernel writes.
-template <typename S>
-__device__ __forceinline__ S load_uniform(const S* p)
-{
-    static_assert(sizeof(S) % 4 == 0, "whole dwords");
-    constexpr int n = (int)(sizeof(S) / 4);
-    const __attribute__((address_space(4))) uint32_t* q = (const __attribute__((address_space(4))) uint32_t*)p;
-    uint32_t w[n];
-#pragma unroll
-    for (int i = 0; i < n; i++) w[i] = q[i];
-    S out;
-    __builtin_memcpy(&out, w, sizeof(S));
-    return out;
-}
-
 // Orders the wave's LDS accesses across lanes (the zeroing and record
 // loops touch other lanes' plane sums): a wavefront-scope fence that the
 // compiler does not move memory operations across.

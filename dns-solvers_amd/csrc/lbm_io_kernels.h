@@ -18,6 +18,11 @@ template <>
 __device__ __forceinline__ double from_f<double>(double f, int) { return f; }
 template <>
 __device__ __forceinline__ float from_f<float>(double f, int a) { return (float)(f - WD[a]); }
+template <>
+__device__ __forceinline__ _Float16 from_f<_Float16>(double f, int a)
+{
+    return Store<_Float16>::from_g((float)(f - WD[a]));
+}
 
 // Populations of the current post-stream state of node (col, z) as f (FP64).
 template <typename T>

@@ -61,6 +61,7 @@ struct StepArgs {
     const int8_t* mtop;     // [X*Y] top wall codes
     const uint32_t* wword;  // [X*Y] per-column wall words
     const uint32_t* bands;  // [X*Y] solid bands hb | ht << 16 (null: flat walls only)
+    const uint32_t* bnbr;   // [X*Y][16] the bands of each column's 3x3 neighbourhood (band_nbr_kernel)
     double omega;
     const double* G;      // force used by this step (device scalar)
     const double* shift;  // [3][Z] plane-mean shifts for the one-pass variances
@@ -80,6 +81,19 @@ template <>
 struct Store<float> {
     // FP32: shifted populations g = f - w.
     static __device__ __forceinline__ double to_f(float v, int a) { return WD[a] + (double)v; }
+    static __device__ __forceinline__ float to_g(float v) { return v; }
+    static __device__ __forceinline__ float from_g(float g) { return g; }
+};
+// FP16 storage of the shifted populations, scaled by 2^16 (exact power of
+// two) so that deviations down to 1e-9 stay normal numbers while |g| < 1
+// stays below the FP16 maximum; arithmetic in FP32.
+constexpr float kHalfScale = 65536.0f;
+constexpr float kHalfInv = 1.0f / 65536.0f;
+template <>
+struct Store<_Float16> {
+    static __device__ __forceinline__ float to_g(_Float16 v) { return (float)v * kHalfInv; }
+    static __device__ __forceinline__ _Float16 from_g(float g) { return (_Float16)(g * kHalfScale); }
+    static __device__ __forceinline__ double to_f(_Float16 v, int a) { return WD[a] + (double)to_g(v); }
 };
 
 // Column geometry of one work item.
@@ -154,6 +168,29 @@ __device__ __forceinline__ uint32_t load_uniform_u32(const uint32_t* p)
 {
     return *(const __attribute__((address_space(4))) uint32_t*)p;
 }
+
+// Wave-uniform load through the constant address space (scalar loads) of
+// a table entry that no kernel running at the same time writes.
+template <typename S>
+__device__ __forceinline__ S load_uniform(const S* p)
+{
+    static_assert(sizeof(S) % 4 == 0, "whole dwords");
+    constexpr int n = (int)(sizeof(S) / 4);
+    const __attribute__((address_space(4))) uint32_t* q = (const __attribute__((address_space(4))) uint32_t*)p;
+    uint32_t w[n];
+#pragma unroll
+    for (int i = 0; i < n; i++) w[i] = q[i];
+    S out;
+    __builtin_memcpy(&out, w, sizeof(S));
+    return out;
+}
+
+// A column's band neighbourhood, one 64-byte record read with one scalar
+// load: w[3*i + j] = bands of column (x - ex, y - ey) for i = ex + 1,
+// j = ey + 1; w[9] = max hb | max ht << 16 over the nine.
+struct alignas(64) BandNbr {
+    uint32_t w[16];
+};
 
 // Solid bands inside the fluid rows (riblets): column (x, y) is solid for
 // rows z < hb and z >= Z - ht, packed as hb | ht << 16.  No reference
@@ -304,6 +341,25 @@ __host__ __device__ inline int step_waves(int Z)
     return w;
 }
 
+// FP32 collision of stored shifted populations, in place.  FP16 storage
+// converts in and out (the stored, rounded values are the next state).
+__device__ __forceinline__ void collide_stored(float (&v)[Q], float omega, float G, float& drho, float& u0,
+                                               float& u1, float& u2, float& jx)
+{
+    collide_fp32(v, omega, G, drho, u0, u1, u2, jx);
+}
+
+__device__ __forceinline__ void collide_stored(_Float16 (&v)[Q], float omega, float G, float& drho, float& u0,
+                                               float& u1, float& u2, float& jx)
+{
+    float g[Q];
+#pragma unroll
+    for (int a = 0; a < Q; a++) g[a] = Store<_Float16>::to_g(v[a]);
+    collide_fp32(g, omega, G, drho, u0, u1, u2, jx);
+#pragma unroll
+    for (int a = 0; a < Q; a++) v[a] = Store<_Float16>::from_g(g[a]);
+}
+
 // Register accumulators of one wave over the columns it updates.
 struct StepAcc {
     double jx = 0.0, mass = 0.0, mom = 0.0, sq = 0.0;
@@ -335,15 +391,13 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
     uint32_t bw[3][3];
     int maxhb = 0, maxht = 0;
     if constexpr (BANDS) {
-        const int sx[3] = {c.xp, c.x, c.xm}, sy[3] = {c.yp, c.y, c.ym};
+        const BandNbr nb = load_uniform(reinterpret_cast<const BandNbr*>(A.bnbr) + col);
 #pragma unroll
         for (int i = 0; i < 3; i++)
 #pragma unroll
-            for (int j = 0; j < 3; j++) {
-                bw[i][j] = load_uniform_u32(A.bands + sx[i] * Y + sy[j]);
-                maxhb = max(maxhb, (int)(bw[i][j] & 0xFFFFu));
-                maxht = max(maxht, (int)(bw[i][j] >> 16));
-            }
+            for (int j = 0; j < 3; j++) bw[i][j] = nb.w[3 * i + j];
+        maxhb = (int)(nb.w[9] & 0xFFFFu);
+        maxht = (int)(nb.w[9] >> 16);
     }
     for (int ch = 0; ch < nzc; ch++) {
         const int z = ch * kWave + lane;
@@ -389,7 +443,7 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
             for (int a = 0; a < Q; a++) gx[a] = v[a];
         } else {
             float drho, u0, u1, u2, jx;
-            collide_fp32(v, (float)A.omega, (float)G, drho, u0, u1, u2, jx);
+            collide_stored(v, (float)A.omega, (float)G, drho, u0, u1, u2, jx);
             jx_raw = (double)jx;
             mass_node = (double)drho;
             if constexpr (STATS) {
@@ -404,7 +458,7 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
                 }
             }
 #pragma unroll
-            for (int a = 0; a < Q; a++) gx[a] = WD[a] + (double)v[a];
+            for (int a = 0; a < Q; a++) gx[a] = WD[a] + (double)Store<T>::to_g(v[a]);
         }
         g11 = gx[11];
         g12 = gx[12];
@@ -554,6 +608,28 @@ static __global__ __launch_bounds__(256) void wall_word_kernel(const int8_t* __r
     ww[col] = w;
 }
 
+// Band neighbourhood records (BandNbr) of every column.
+static __global__ __launch_bounds__(256) void band_nbr_kernel(const uint32_t* __restrict__ bands, int X, int Y,
+                                                             uint32_t* __restrict__ out)
+{
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= X * Y) return;
+    const int x = col / Y, y = col - x * Y;
+    const int sx[3] = {x == X - 1 ? 0 : x + 1, x, x == 0 ? X - 1 : x - 1};
+    const int sy[3] = {y == Y - 1 ? 0 : y + 1, y, y == 0 ? Y - 1 : y - 1};
+    uint32_t* r = out + (size_t)col * 16;
+    uint32_t mb = 0, mt = 0;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            const uint32_t w = bands[sx[i] * Y + sy[j]];
+            r[3 * i + j] = w;
+            mb = max(mb, w & 0xFFFFu);
+            mt = max(mt, w >> 16);
+        }
+    r[9] = mb | (mt << 16);
+    for (int k = 10; k < 16; k++) r[k] = 0u;
+}
+
 // Per-lane accumulators of the chunk-mapped kernels (one wave per
 // (column, 64-row chunk)): per-plane sums over the cpb column slots in
 // fixed order, global sums in a fixed order, written as partials.
@@ -634,7 +710,7 @@ static __global__ __launch_bounds__(kWave* kMaxWaves) void moments_kernel(StepAr
         } else {
             double dr = 0.0;
 #pragma unroll
-            for (int a = 0; a < Q; a++) dr += (double)v[a];
+            for (int a = 0; a < Q; a++) dr += (double)Store<T>::to_g(v[a]);
             acc[3] += dr;
         }
         acc[kNQ + 0] += ux;  // jx

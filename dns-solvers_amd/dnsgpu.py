@@ -30,7 +30,7 @@ EXPORTS = (
     "dnsgpu_footprint",
     "dnsens_create", "dnsens_destroy", "dnsens_member_bytes", "dnsens_footprint", "dnsens_add", "dnsens_remove",
     "dnsens_count", "dnsens_bytes_used", "dnsens_budget", "dnsens_step", "dnsens_info", "dnsens_rows",
-    "dnsens_autotune", "dnsens_time_steps", "dnsens_grid",
+    "dnsens_autotune", "dnsens_time_steps", "dnsens_time_kernels", "dnsens_grid",
 )
 
 ENS_NO_GRAPH, ENS_AUTOTUNE = 1, 2
@@ -148,6 +148,7 @@ def load():
         L.dnsens_rows.argtypes = [vp, vp, ctypes.POINTER(DnsStepLog), ctypes.c_int]
         L.dnsens_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), dp]
         L.dnsens_time_steps.argtypes = [vp, ctypes.c_int, dp]
+        L.dnsens_time_kernels.argtypes = [vp, ctypes.c_int, dp, dp]
         L.dnsens_grid.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
         _lib = L
     return _lib
@@ -402,6 +403,13 @@ class Ensemble:
         t = ctypes.c_double()
         _chk(load().dnsens_time_steps(self.e, n, ctypes.byref(t)))
         return t.value
+
+    def time_kernels(self, n):
+        """(step-kernel ms summed over n steps, total ms) with HIP events."""
+        tk = ctypes.c_double()
+        tw = ctypes.c_double()
+        _chk(load().dnsens_time_kernels(self.e, n, ctypes.byref(tk), ctypes.byref(tw)))
+        return tk.value, tw.value
 
     def grid(self):
         b = ctypes.c_int()

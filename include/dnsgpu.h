@@ -297,6 +297,11 @@ int dnsens_autotune(dnsens *e, int trial_steps, int *best_blocks, double *best_m
  * (HIP events on the ensemble's stream) in ms. */
 int dnsens_time_steps(dnsens *e, int nsteps, double *ms);
 
+/* Advance nsteps steps with direct launches, timing each step kernel with
+ * HIP events on the ensemble's stream: ms_step_kernels receives the sum of
+ * the step-kernel times, ms_total the whole sequence (may be NULL). */
+int dnsens_time_kernels(dnsens *e, int nsteps, double *ms_step_kernels, double *ms_total);
+
 /* Current step-kernel grid (blocks) and tile count. */
 int dnsens_grid(const dnsens *e, int *blocks, int *tiles);
 
