@@ -9,7 +9,7 @@ reductions, with the state resident in HBM.
 Default workload: the Re_tau = 180 smooth-wall channel, grid
 768 x 384 x 192 (xdv streamwise, ydv spanwise, zdv wall-normal; 56.6 M
 fluid nodes), constant body force, plain FP64 populations (the value line)
-and FP32-shifted populations (reported beside it).
+and FP32-shifted and FP16-shifted populations (reported beside it).
 
 Prints ONE JSON line (rank 0).  Multi-GPU runs are not part of this build.
 """
@@ -90,7 +90,7 @@ def run_precision(wname, prec, steps, warmup):
         ok = all(v == v for v in logs["stress"])
     finally:
         ch.close()
-    esize = 8 if prec == dnsgpu.FP64 else 4
+    esize = {dnsgpu.FP64: 8, dnsgpu.FP32: 4, dnsgpu.FP16: 2}[prec]
     bytes_per_node = 2 * 19 * esize
     avg_kernel_s = ms_k / 1e3 / steps
     achieved = bytes_per_node * nodes / avg_kernel_s / 1e9
@@ -145,6 +145,7 @@ def measured_traffic(wname):
 
 
 SWEEP = "texture_sweep_64"
+PNAME = {dnsgpu.FP64: "fp64", dnsgpu.FP32: "fp32", dnsgpu.FP16: "fp16"}
 
 # Drag reduction of a textured channel against the smooth one at the same
 # constant body force, FP64 and FP32-shifted from the same seeded field:
@@ -173,7 +174,7 @@ def run_drag(name, args):
     steps = args.dr_steps or d["steps"]
     avg_from = min(args.dr_avg_from or d["avg_from"], steps - 500)
     runs = {}
-    for prec in (dnsgpu.FP64, dnsgpu.FP32):
+    for prec in (dnsgpu.FP64, dnsgpu.FP32, dnsgpu.FP16):
         for textured in (False, True):
             ch = dnsgpu.Channel(X, Y, Z, Rebs=ub * Z / nu, ub=ub, CFL=0.1, TSTR=0, forcing=dnsgpu.CONST_F,
                                 force=F, precision=prec, omega=omega)
@@ -200,16 +201,16 @@ def run_drag(name, args):
             finally:
                 ch.close()
             runs[(prec, textured)] = (tv, wall)
-            print(f"# {'fp64' if prec == dnsgpu.FP64 else 'fp32'} {'textured' if textured else 'smooth'}: "
+            print(f"# {PNAME[prec]} {'textured' if textured else 'smooth'}: "
                   f"Ub={tv['Ub']:.6e} {wall:.1f} s", file=sys.stderr, flush=True)
 
     def dr(prec):
         return dnsgpu.drag_reduction(runs[(prec, False)][0], runs[(prec, True)][0], dnsgpu.CONST_F)["DR_Ub"]
 
-    t64, t32 = runs[(dnsgpu.FP64, True)][0], runs[(dnsgpu.FP32, True)][0]
-    u64, u32 = t64["profiles"][0], t32["profiles"][0]
-    prof_err = float(np.abs(u32 - u64).max() / np.abs(u64).max())
-    ut64, ut32 = np.sqrt(abs(t64["stress"])), np.sqrt(abs(t32["stress"]))
+    t64 = runs[(dnsgpu.FP64, True)][0]
+    u64 = t64["profiles"][0]
+    ut64 = np.sqrt(abs(t64["stress"]))
+    slip64 = 0.5 * (t64["u_slip_bottom"] + t64["u_slip_top"])
     nodes = X * Y * Z
     out = {
         "metric": "MLUPS + % HBM roofline (FP32 vs FP64 D3Q19); ensemble solves/h + p99 latency",
@@ -220,21 +221,25 @@ def run_drag(name, args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": name, "grid_xdv_ydv_zdv": [X, Y, Z], "texture": "%s %d %d" % d["texture"],
                    "forcing": "constant body force", "averaging_window": [avg_from, steps],
+                   "drag_reduction": "1 - (Ub_smooth / Ub_textured)^2 at constant force",
                    "parallelism": "single GPU"},
-        "drag_reduction_percent": {"fp64": round(dr(dnsgpu.FP64), 4), "fp32_shifted": round(dr(dnsgpu.FP32), 4),
-                                   "error_points": round(abs(dr(dnsgpu.FP32) - dr(dnsgpu.FP64)), 4),
-                                   "definition": "1 - (Ub_smooth / Ub_textured)^2 at constant force"},
-        "mean_profile_max_rel_error": prof_err,
-        "u_tau_wall_stress": {"fp64": ut64, "fp32_shifted": ut32, "rel_error": abs(ut32 - ut64) / ut64},
-        "slip_velocity": {"fp64": 0.5 * (t64["u_slip_bottom"] + t64["u_slip_top"]),
-                          "fp32_shifted": 0.5 * (t32["u_slip_bottom"] + t32["u_slip_top"])},
-        "Ub": {f"{'fp64' if p == dnsgpu.FP64 else 'fp32'}_{'textured' if t else 'smooth'}": runs[(p, t)][0]["Ub"]
-               for (p, t) in runs},
-        "wall_s": {f"{'fp64' if p == dnsgpu.FP64 else 'fp32'}_{'textured' if t else 'smooth'}": round(runs[(p, t)][1], 2)
-                   for (p, t) in runs},
+        "fp64": {"drag_reduction_percent": round(dr(dnsgpu.FP64), 4), "u_tau_wall_stress": ut64,
+                 "slip_velocity": slip64},
+        "Ub": {f"{PNAME[p]}_{'textured' if t else 'smooth'}": runs[(p, t)][0]["Ub"] for (p, t) in runs},
+        "wall_s": {f"{PNAME[p]}_{'textured' if t else 'smooth'}": round(runs[(p, t)][1], 2) for (p, t) in runs},
     }
-    sv = out["slip_velocity"]
-    sv["rel_error"] = abs(sv["fp32_shifted"] - sv["fp64"]) / abs(sv["fp64"]) if sv["fp64"] else None
+    for prec in (dnsgpu.FP32, dnsgpu.FP16):
+        t = runs[(prec, True)][0]
+        ut = np.sqrt(abs(t["stress"]))
+        slip = 0.5 * (t["u_slip_bottom"] + t["u_slip_top"])
+        out[PNAME[prec] + "_shifted"] = {
+            "drag_reduction_percent": round(dr(prec), 4),
+            "drag_reduction_error_points": round(abs(dr(prec) - dr(dnsgpu.FP64)), 4),
+            "mean_profile_max_rel_error": float(np.abs(t["profiles"][0] - u64).max() / np.abs(u64).max()),
+            "u_tau_wall_stress_rel_error": abs(ut - ut64) / ut64,
+            "slip_velocity": slip,
+            "slip_velocity_rel_error": abs(slip - slip64) / abs(slip64) if slip64 else None,
+        }
     return out
 
 
@@ -247,7 +252,8 @@ def run_sweep(args):
     the plain FP64 single-case path."""
     import dnsserve
     reqs = dnsserve.generate(64, seed=5, mean_gap=2.0)
-    cfg = dnsserve.defaults(realtime=0 if args.burst else 1, verbose=1)
+    prec = dnsgpu.FP16 if args.sweep_fp16 else dnsgpu.FP32
+    cfg = dnsserve.defaults(realtime=0 if args.burst else 1, verbose=1, precision=prec)
     res, summ = dnsserve.sweep(reqs, cfg)
     print(f"# sweep done: {summ['wall']:.1f} s, {summ['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
     steps = [r["steps"] if r["status"] >= 0 else 0 for r in res]
@@ -257,7 +263,7 @@ def run_sweep(args):
         print(f"# sequential fp64 done: {seq:.1f} s", file=sys.stderr, flush=True)
     # the dominant kernel: the multi-case step with all 64 solves resident
     hs = [dnsserve.Handle(r, cfg) for r in reqs]
-    ens = dnsgpu.Ensemble(precision=dnsgpu.FP32)
+    ens = dnsgpu.Ensemble(precision=prec)
     for h in hs:
         ens.add(h)
     blocks, _ = ens.autotune(3)
@@ -268,7 +274,8 @@ def run_sweep(args):
     for h in hs:
         h.close()
     nodes = sum(r.xdv * r.ydv * r.zdv for r in reqs)
-    achieved = nodes * 152 / (tk / n / 1e3) / 1e9
+    bpn = 152 if prec == dnsgpu.FP32 else 76
+    achieved = nodes * bpn / (tk / n / 1e3) / 1e9
     status = {str(k): sum(1 for r in res if r["status"] == k) for k in (1, 0, -1, -2)}
     out = {
         "metric": "MLUPS + % HBM roofline (FP32 vs FP64 D3Q19); ensemble solves/h + p99 latency",
@@ -281,9 +288,9 @@ def run_sweep(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32",
+        "dtype": "f32" if prec == dnsgpu.FP32 else "f32 (f16 storage)",
         "data": "synthetic",
-        "config": {"workload": SWEEP, "solves": 64, "seed": 5,
+        "config": {"workload": SWEEP, "solves": 64, "seed": 5, "storage": PNAME[prec] + "-shifted",
                    "arrivals": "all at t=0" if args.burst else "Poisson, mean gap 2 s",
                    "grids": "{128,256,512} x {64,128,256} x {64,96,128} (xdv x ydv x zdv)",
                    "re_tau": "U[90,180]", "textures": "grooves (period 8-64, gas fraction 0.3-0.9) or riblets "
@@ -303,7 +310,8 @@ def run_sweep(args):
         "speedup_vs_sequential_fp64": None if seq is None else round(seq / summ["wall"], 3),
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
-                     "kernel": "ens_step_kernel<float>", "kernel_ms": round(tk / n, 4), "bytes_per_node": 152,
+                     "kernel": "ens_step_kernel<%s>" % ("float" if prec == dnsgpu.FP32 else "_Float16"),
+                     "kernel_ms": round(tk / n, 4), "bytes_per_node": bpn,
                      "nodes": nodes, "grid_blocks": blocks},
         "solves": res,
     }
@@ -324,6 +332,7 @@ def main():
     ap.add_argument("--fp64-only", action="store_true")
     ap.add_argument("--burst", action="store_true", help=f"{SWEEP}: every request at t = 0")
     ap.add_argument("--no-sequential", action="store_true", help=f"{SWEEP}: skip the sequential FP64 solves")
+    ap.add_argument("--sweep-fp16", action="store_true", help=f"{SWEEP}: FP16-shifted storage")
     ap.add_argument("--dr-steps", type=int, default=0, help="drag workloads: steps per run")
     ap.add_argument("--dr-avg-from", type=int, default=0, help="drag workloads: first averaged step")
     args = ap.parse_args()
@@ -340,10 +349,12 @@ def main():
         return
     r64 = run_precision(args.workload, dnsgpu.FP64, args.steps, args.warmup)
     print(f"# fp64 done: {r64['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
-    r32 = None
+    r32 = r16 = None
     if not args.fp64_only:
         r32 = run_precision(args.workload, dnsgpu.FP32, args.steps, args.warmup)
         print(f"# fp32 done: {r32['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
+        r16 = run_precision(args.workload, dnsgpu.FP16, args.steps, args.warmup)
+        print(f"# fp16 done: {r16['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(args.workload)
     w = WORKLOADS[args.workload]
     tr = measured_traffic(args.workload)
@@ -380,6 +391,12 @@ def main():
                                              "peak": PEAK_HBM_GBS, "unit": "GB/s",
                                              "frac": round(r32["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": t32,
                                              "kernel": "step_kernel<float,false,true>"})
+    if r16:
+        out["fp16_shifted"] = dict(r16, speedup_vs_fp64=round(r16["mlups"] / r64["mlups"], 3),
+                                   roofline={"bound": "hbm", "achieved": round(r16["achieved_gbs"], 1),
+                                             "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                                             "frac": round(r16["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": None,
+                                             "kernel": "step_kernel<_Float16,false,true>"})
     if cpu:
         out["cpu_baseline"] = cpu
     print(json.dumps(out))

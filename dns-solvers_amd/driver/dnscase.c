@@ -95,6 +95,7 @@ static int set_key(dns_run_case *c, const char *k, const char *v, char *err, int
     if (!strcmp(k, "precision")) {
         if (!strcmp(v, "fp64")) c->precision = 0;
         else if (!strcmp(v, "fp32")) c->precision = 1;
+        else if (!strcmp(v, "fp16")) c->precision = 2;
         else goto bad;
         return 0;
     }

@@ -8,7 +8,7 @@
  *
  *   forcing    = constq | constf      (default constq, encalc.c:121)
  *   force      = <double>             body force for constf
- *   precision  = fp64 | fp32          (default fp64)
+ *   precision  = fp64 | fp32 | fp16   (default fp64; fp32/fp16: shifted storage)
  *   omega      = <double>             1/tau; default from Rebs, ub, zdv
  *   texture    = smooth | posts | stripes | riblets  (PSLIP / RSLIP set posts / stripes)
  *   riblet_spacing, riblet_height, riblet_offset   (triangular riblets, cells)
@@ -34,7 +34,7 @@ typedef struct dns_run_case {
     /* build keys */
     int forcing;   /* 0 constant Q, 1 constant F */
     double force;
-    int precision; /* 0 fp64, 1 fp32 */
+    int precision; /* 0 fp64, 1 fp32, 2 fp16 (shifted storage) */
     double omega;  /* 0 => derived */
     int texture;   /* 0 smooth, 1 posts, 2 stripes, 3 riblets */
     double riblet_spacing, riblet_height, riblet_offset;

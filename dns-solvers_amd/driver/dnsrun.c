@@ -232,7 +232,7 @@ static void write_report(const dns_run_case *c, double omega, double seconds, do
     fprintf(tm, "Unit non-dimensional time is: %f iterations\n", utime);
     fprintf(tm, "Starting time: %d\n", c->TSTR);
     fprintf(tm, "Current Time is: %d\n", c->TEND);
-    fprintf(tm, "Device: MI355X (gfx950), precision: %s, forcing: %s\n", c->precision ? "fp32-shifted" : "fp64",
+    fprintf(tm, "Device: MI355X (gfx950), precision: %s, forcing: %s\n", c->precision == 2 ? "fp16-shifted" : (c->precision ? "fp32-shifted" : "fp64"),
             c->forcing ? "constant force" : "constant flow rate");
     fclose(tm);
 }

@@ -10,7 +10,8 @@ Bars:
   * constant-Q FP64: populations within 1e-12 relative of the single-case
     path; log rows within 1e-10.
   * alone == in the mix: bitwise (populations, log rows, time averages) in
-    FP64 and FP32, for members admitted and retired at different steps.
+    FP64, FP32 and FP16 storage, for members admitted and retired at
+    different steps.
   * graph replays: bitwise equal to the same launches made directly.
 """
 import numpy as np
@@ -82,7 +83,7 @@ def test_members_match_single_case_path(forcing):
     ens.close()
 
 
-@pytest.mark.parametrize("precision", [dnsgpu.FP64, dnsgpu.FP32])
+@pytest.mark.parametrize("precision", [dnsgpu.FP64, dnsgpu.FP32, dnsgpu.FP16])
 def test_alone_equals_in_mix(precision):
     """A member's result does not depend on the other members: the same
     solve run alone and in a changing mix (admissions and retirements at
