@@ -1,12 +1,12 @@
 /* dnsserve -- serve the seeded texture sweep through one ensemble and
  * print one JSON object per solve and a summary line.
  *
- *   dnsserve [-n solves] [-s seed] [-g mean_gap_s] [-b] [-p fp32|fp64]
+ *   dnsserve [-n solves] [-s seed] [-g mean_gap_s] [-b] [-p fp32|fp16|fp64]
  *            [-c cap] [-w window] [-t transient] [-k check] [-q seq_precision]
  *
  *   -b   burst: every request arrives at t = 0 (no arrival times)
  *   -q   also run the same solves one after another on the single-case
- *        path at this precision (fp64 or fp32) and report their time
+ *        path at this precision (fp64, fp32 or fp16) and report their time
  */
 #define _POSIX_C_SOURCE 200809L
 #include <stdio.h>
@@ -16,7 +16,10 @@
 
 #include "dnsserve.h"
 
-static int prec_of(const char *s) { return !strcmp(s, "fp64") ? DNSGPU_FP64 : DNSGPU_FP32; }
+static int prec_of(const char *s)
+{
+    return !strcmp(s, "fp64") ? DNSGPU_FP64 : (!strcmp(s, "fp16") ? DNSGPU_FP16 : DNSGPU_FP32);
+}
 
 int main(int argc, char **argv)
 {
@@ -39,7 +42,7 @@ int main(int argc, char **argv)
         case 'k': cfg.check = atoi(optarg); break;
         case 'q': seq = prec_of(optarg); break;
         default:
-            fprintf(stderr, "usage: %s [-n solves] [-s seed] [-g gap] [-b] [-p fp32|fp64] [-c cap] [-w window] "
+            fprintf(stderr, "usage: %s [-n solves] [-s seed] [-g gap] [-b] [-p fp32|fp16|fp64] [-c cap] [-w window] "
                             "[-t transient] [-k check] [-q fp64|fp32]\n", argv[0]);
             return 2;
         }
