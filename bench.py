@@ -257,10 +257,21 @@ def run_sweep(args):
     res, summ = dnsserve.sweep(reqs, cfg)
     print(f"# sweep done: {summ['wall']:.1f} s, {summ['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
     steps = [r["steps"] if r["status"] >= 0 else 0 for r in res]
-    seq = None
+    seq = fifo = None
     if not args.no_sequential:
-        seq = sum(dnsserve.sequential(reqs, cfg, dnsgpu.FP64, steps))
+        secs = dnsserve.sequential(reqs, cfg, dnsgpu.FP64, steps)
+        seq = sum(secs)
         print(f"# sequential fp64 done: {seq:.1f} s", file=sys.stderr, flush=True)
+        # the same requests served one at a time in arrival order by the FP64
+        # single-case path (a queue alone), from the measured per-solve times
+        t, ttr = 0.0, []
+        for r, sec in zip(reqs, secs):
+            arr = r.arrival if not args.burst else 0.0
+            t = max(t, arr) + sec
+            ttr.append(t - arr)
+        ttr.sort()
+        k = lambda p: ttr[max(0, min(len(ttr) - 1, int(-(-p * len(ttr) // 100)) - 1))]
+        fifo = {"wall_s": round(t, 3), "p50_s": round(k(50), 3), "p99_s": round(k(99), 3)}
     # the dominant kernel: the multi-case step with all 64 solves resident
     hs = [dnsserve.Handle(r, cfg) for r in reqs]
     ens = dnsgpu.Ensemble(precision=prec)
@@ -308,6 +319,7 @@ def run_sweep(args):
         "status_counts": status,
         "sequential_fp64_s": None if seq is None else round(seq, 3),
         "speedup_vs_sequential_fp64": None if seq is None else round(seq / summ["wall"], 3),
+        "sequential_fp64_queue": fifo,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
                      "kernel": "ens_step_kernel<%s>" % ("float" if prec == dnsgpu.FP32 else "_Float16"),
