@@ -103,6 +103,14 @@ def test_case_file_and_reference_textures(lib, tmp_path):
     lib.dns_case_defaults(ctypes.byref(c))
     assert lib.dns_case_read(str(cf).encode(), ctypes.byref(c), err, 512) == 0, err.value
     assert (c.xdv, c.ydv, c.zdv, c.TSTR, c.ITER, c.texture, c.forcing, c.precision) == (12, 8, 10, 5, 40, 1, 1, 1)
+    for name, code in (("fp64", 0), ("fp32", 1), ("fp16", 2)):
+        cf.write_text(f"precision = {name}\ntavg_start = 100\ntavg_every = 20\n")
+        c2 = RunCase()
+        lib.dns_case_defaults(ctypes.byref(c2))
+        assert lib.dns_case_read(str(cf).encode(), ctypes.byref(c2), err, 512) == 0, err.value
+        assert (c2.precision, c2.tavg_start, c2.tavg_every) == (code, 100, 20)
+    cf.write_text("precision = fp8\n")
+    assert lib.dns_case_read(str(cf).encode(), ctypes.byref(c2), err, 512) == -1
     b = np.zeros((12, 8), np.int8)
     t = np.zeros((12, 8), np.int8)
     lib.dns_case_masks(ctypes.byref(c), b.ctypes.data, t.ctypes.data)
