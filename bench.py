@@ -372,6 +372,7 @@ def main():
     tr = measured_traffic(args.workload)
     t64 = tr["fp64_stats"]["traffic_bytes_per_launch"] if tr else None
     t32 = tr["fp32_stats"]["traffic_bytes_per_launch"] if tr else None
+    t16 = tr["fp16_stats"]["traffic_bytes_per_launch"] if tr and "fp16_stats" in tr else None
     out = {
         "metric": "MLUPS + % HBM roofline (FP32 vs FP64 D3Q19); ensemble solves/h + p99 latency",
         "value": round(r64["mlups"], 1),
@@ -407,7 +408,7 @@ def main():
         out["fp16_shifted"] = dict(r16, speedup_vs_fp64=round(r16["mlups"] / r64["mlups"], 3),
                                    roofline={"bound": "hbm", "achieved": round(r16["achieved_gbs"], 1),
                                              "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                                             "frac": round(r16["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": None,
+                                             "frac": round(r16["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": t16,
                                              "kernel": "step_kernel<_Float16,false,true>"})
     if cpu:
         out["cpu_baseline"] = cpu

@@ -437,7 +437,7 @@ extern "C" int dnsgpu_autotune(dnsgpu* h, int trial_steps, int* best_blocks, dou
     }
     const int saved_ps = h->poststream;
     h->poststream = 0;  // time the steady-state kernel
-    const int wpcs[] = {2, 3, 4, 6, 8, 12, 16};
+    const int wpcs[] = {2, 3, 4, 6, 8, 12, 16, 20, 24, 32};
     int best = h->step_nblk;
     double best_t = 1e30;
     int last = -1;
