@@ -69,6 +69,7 @@ StepArgs step_args(dnsgpu* h, const void* src, void* dst, int nblk)
     A.wword = h->wword;
     A.bands = h->has_bands ? h->bands : nullptr;
     A.bnbr = h->has_bands ? h->bnbr : nullptr;
+    A.cnbr = h->cnbr;
     A.omega = h->omega;
     A.G = &h->st->G;
     A.shift = h->shift;
@@ -290,6 +291,7 @@ static std::vector<DevAlloc> handle_allocs(dnsgpu* h)
         {(void**)&h->wword, xy * 4},
         {(void**)&h->bands, xy * 4},
         {(void**)&h->bnbr, xy * sizeof(BandNbr)},
+        {(void**)&h->cnbr, xy * sizeof(ColNbr)},
         {(void**)&h->nf, sizeof(double) * g.Z},
         {(void**)&h->part, sizeof(double) * kNStat * g.Z * np},
         {(void**)&h->gpart, sizeof(double) * kNG * np},
@@ -346,6 +348,9 @@ extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
         hipMemsetAsync(h->shift, 0, sizeof(double) * 3 * g.Z, h->stream) != hipSuccess)
         return fail(set_err(DNSGPU_EHIP, "initial memset failed"));
     if (update_wall_words(h)) return fail(DNSGPU_EHIP);
+    hipLaunchKernelGGL(column_nbr_kernel, dim3((unsigned)((xy + 255) / 256)), dim3(256), 0, h->stream, g.X, g.Y,
+                       h->cnbr);
+    if (hipGetLastError() != hipSuccess) return fail(set_err(DNSGPU_EHIP, "column records"));
     // the pop buffers start at rest
     h->cur = 0;
     h->poststream = 1;
@@ -362,7 +367,7 @@ extern "C" void dnsgpu_destroy(dnsgpu* h)
     if (!h) return;
     if (h->owner) (void)dnsens_remove(h->owner, h);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
-    void* ptrs[] = {h->buf[0], h->buf[1], h->mbot,  h->mtop,  h->wword, h->bands, h->bnbr, h->nf,
+    void* ptrs[] = {h->buf[0], h->buf[1], h->mbot,  h->mtop,  h->wword, h->bands, h->bnbr, h->cnbr, h->nf,
                     h->part,   h->gpart,  h->planes, h->shift, h->means, h->tprof, h->slip,
                     h->st,     h->ring,   h->stage};
     for (void* p : ptrs)

@@ -37,6 +37,7 @@ struct dnsgpu {
     uint32_t* wword;
     uint32_t* bands;  // [X*Y] solid band words hb | ht << 16
     uint32_t* bnbr;   // [X*Y][16] band neighbourhood records
+    uint32_t* cnbr;   // [X*Y][16] column neighbourhood records
     double* nf;       // [Z] fluid nodes per plane
     double nfluid;    // fluid nodes in total
     int has_bands;

@@ -62,6 +62,7 @@ struct StepArgs {
     const uint32_t* wword;  // [X*Y] per-column wall words
     const uint32_t* bands;  // [X*Y] solid bands hb | ht << 16 (null: flat walls only)
     const uint32_t* bnbr;   // [X*Y][16] the bands of each column's 3x3 neighbourhood (band_nbr_kernel)
+    const uint32_t* cnbr;   // [X*Y][16] each column's 3x3 neighbour indices (column_nbr_kernel)
     double omega;
     const double* G;      // force used by this step (device scalar)
     const double* shift;  // [3][Z] plane-mean shifts for the one-pass variances
@@ -144,6 +145,13 @@ __device__ __forceinline__ long long colbase(const Column& c)
     const int sy = ey == 1 ? c.ym : (ey == -1 ? c.yp : c.y);
     return (long long)(sx * c.Y + sy) * Q * c.Z;
 }
+
+// A column's neighbourhood, one 64-byte record read with one scalar load:
+// w[3*i + j] = index of column (x - ex, y - ey) for i = ex + 1, j = ey + 1
+// (periodic wrap in x and y), precomputed once per grid (column_nbr_kernel).
+struct alignas(64) ColNbr {
+    uint32_t w[16];
+};
 
 // Wall word of a column: the wall rule of its two wall-adjacent nodes,
 // precomputed from the wall masks (wall_word_kernel).
@@ -280,24 +288,24 @@ __device__ __forceinline__ void load_local19(const T* __restrict__ src, const Co
 // cache line is fetched) and are then overwritten by the wall rule: uniform
 // decisions from the column's wall word, one active lane.
 template <typename T, bool FIRST>
-__device__ __forceinline__ void gather_wave(const T* __restrict__ src, const Column& c, uint32_t zoff,
-                                            uint32_t zoff_up, uint32_t zoff_dn, bool bot_lane, bool top_lane,
-                                            int z, uint32_t ww, T (&v)[Q])
+__device__ __forceinline__ void gather_wave(const char* const (&sp)[3][3], int Z, uint32_t zoff, uint32_t zoff_up,
+                                            uint32_t zoff_dn, bool bot_lane, bool top_lane, uint32_t ww,
+                                            T (&v)[Q])
 {
-    const char* s8 = (const char*)src;
-    const long long Z = c.Z;
+    // sp[ex + 1][ey + 1]: byte address of the block of column (x - ex, y - ey)
+    const int zb = Z * (int)sizeof(T);  // bytes of one direction segment
     static_for<0, Q>([&](auto ac) {
         constexpr int a = decltype(ac)::value;
-        long long e;
+        const char* p;
         uint32_t off;
         if constexpr (FIRST) {
-            e = c.cb + (long long)a * Z;
+            p = sp[1][1] + a * zb;
             off = zoff;
         } else {
-            e = colbase<EX[a], EY[a]>(c) + (long long)a * Z - EZ[a];
+            p = sp[EX[a] + 1][EY[a] + 1] + (a * zb - EZ[a] * (int)sizeof(T));
             off = EZ[a] == 1 ? zoff_up : (EZ[a] == -1 ? zoff_dn : zoff);
         }
-        v[a] = *(const T*)(s8 + e * (long long)sizeof(T) + off);
+        v[a] = *(const T*)(p + off);
     });
     if constexpr (!FIRST) {
         if (bot_lane) {
@@ -305,9 +313,8 @@ __device__ __forceinline__ void gather_wave(const T* __restrict__ src, const Col
                 constexpr int a = decltype(ac)::value;
                 if constexpr (EZ[a] == 1) {
                     const bool spec = (ww >> wall_bit<a>()) & 1u;
-                    const long long e = spec ? colbase<EX[a], EY[a]>(c) + (long long)MIR[a] * Z
-                                             : c.cb + (long long)OPP[a] * Z;
-                    v[a] = src[e + z];
+                    const char* p = spec ? sp[EX[a] + 1][EY[a] + 1] + MIR[a] * zb : sp[1][1] + OPP[a] * zb;
+                    v[a] = *(const T*)(p + zoff);
                 }
             });
         }
@@ -316,9 +323,8 @@ __device__ __forceinline__ void gather_wave(const T* __restrict__ src, const Col
                 constexpr int a = decltype(ac)::value;
                 if constexpr (EZ[a] == -1) {
                     const bool spec = (ww >> wall_bit<a>()) & 1u;
-                    const long long e = spec ? colbase<EX[a], EY[a]>(c) + (long long)MIR[a] * Z
-                                             : c.cb + (long long)OPP[a] * Z;
-                    v[a] = src[e + z];
+                    const char* p = spec ? sp[EX[a] + 1][EY[a] + 1] + MIR[a] * zb : sp[1][1] + OPP[a] * zb;
+                    v[a] = *(const T*)(p + zoff);
                 }
             });
         }
@@ -381,11 +387,27 @@ template <typename T, bool FIRST, bool STATS, bool BANDS>
 __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane, double G, const double* shl,
                                             double* mine, StepAcc& acc)
 {
-    const int X = A.g.X, Y = A.g.Y, Z = A.g.Z, nzc = A.g.nzc;
+    const int Z = A.g.Z, nzc = A.g.nzc;
     const int ctop = (Z - 1) / kWave, ltop = (Z - 1) % kWave;
     const T* __restrict__ src = (const T*)A.src;
     char* __restrict__ d8 = (char*)A.dst;
-    const Column c = make_column(col, X, Y, Z);
+    // byte addresses of the 3x3 neighbourhood's column blocks in the source
+    // and the destination, [ex + 1][ey + 1] = column (x - ex, y - ey)
+    const char* sp[3][3];
+    char* dp[3][3];
+    {
+        const ColNbr cr = load_uniform(reinterpret_cast<const ColNbr*>(A.cnbr) + col);
+        const uint32_t qzb = (uint32_t)(Q * Z * (int)sizeof(T));  // bytes of a column block (< 2^17)
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                const unsigned long long o = (unsigned long long)cr.w[3 * i + j] * qzb;
+                sp[i][j] = (const char*)src + o;
+                dp[i][j] = d8 + o;
+            }
+    }
+    const int zb = Z * (int)sizeof(T);
     const uint32_t ww = load_uniform_u32(A.wword + col);
     // solid bands of the 3x3 neighbourhood, bw[ex+1][ey+1] = column (x - ex, y - ey)
     uint32_t bw[3][3];
@@ -418,7 +440,7 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
         // a node next to a solid band pulls, like any other, from its
         // neighbour: the solid node's slot holds the bounced-back value
         // (stored there by this node at the previous step, below)
-        gather_wave<T, FIRST>(src, c, zoff, zoff_up, zoff_dn, bot_lane, top_lane, z, ww, v);
+        gather_wave<T, FIRST>(sp, Z, zoff, zoff_up, zoff_dn, bot_lane, top_lane, ww, v);
 
         double mass_node, jx_raw, g11, g12, g13, g14;
         double gx[Q];  // post-collision values of the e_x != 0 directions (budget)
@@ -501,18 +523,14 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
                 const bool inside = zt >= 0 && zt < Z;
                 // x + e_a is not a fluid node (a band node or a wall row)
                 const bool nb_solid = a == 0 ? solid_own : (!inside || band_solid(bw[1 - EX[a]][1 - EY[a]], zt, Z));
-                if (!solid_own || nb_solid)
-                    *(T*)(d8 + (c.cb + (long long)a * Z) * (long long)sizeof(T) + zoff) = v[a];
-                if (a != 0 && !solid_own && nb_solid && inside) {
-                    const long long e = colbase<-EX[a], -EY[a]>(c) + (long long)OPP[a] * Z + EZ[a];
-                    *(T*)(d8 + e * (long long)sizeof(T) + zoff) = v[a];
-                }
+                if (!solid_own || nb_solid) *(T*)(dp[1][1] + a * zb + zoff) = v[a];
+                if (a != 0 && !solid_own && nb_solid && inside)
+                    *(T*)(dp[1 - EX[a]][1 - EY[a]] + (OPP[a] * zb + EZ[a] * (int)sizeof(T)) + zoff) = v[a];
             });
         } else {
             static_for<0, Q>([&](auto ac) {
                 constexpr int a = decltype(ac)::value;
-                const long long e = c.cb + (long long)a * Z;
-                *(T*)(d8 + e * (long long)sizeof(T) + zoff) = v[a];
+                *(T*)(dp[1][1] + a * zb + zoff) = v[a];
             });
         }
     }
@@ -606,6 +624,20 @@ static __global__ __launch_bounds__(256) void wall_word_kernel(const int8_t* __r
         if (mW == -1 && m[xm * Y + y] == -1) w |= 1u << (side ? 13 : 11);
     }
     ww[col] = w;
+}
+
+// Neighbour records (ColNbr) of every column.
+static __global__ __launch_bounds__(256) void column_nbr_kernel(int X, int Y, uint32_t* __restrict__ out)
+{
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= X * Y) return;
+    const int x = col / Y, y = col - x * Y;
+    const int sx[3] = {x == X - 1 ? 0 : x + 1, x, x == 0 ? X - 1 : x - 1};
+    const int sy[3] = {y == Y - 1 ? 0 : y + 1, y, y == 0 ? Y - 1 : y - 1};
+    uint32_t* r = out + (size_t)col * 16;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) r[3 * i + j] = (uint32_t)(sx[i] * Y + sy[j]);
+    for (int k = 9; k < 16; k++) r[k] = 0u;
 }
 
 // Band neighbourhood records (BandNbr) of every column.
