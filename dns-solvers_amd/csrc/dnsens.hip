@@ -272,6 +272,7 @@ static int upload_tables(dnsens* e)
         dnsgpu* h = m.h;
         EnsMember& d = e->h_mem[i];
         d.a = step_args(h, h->buf[h->cur], h->buf[1 - h->cur], m.ntiles);
+        d.a.bnbr = h->bnbr;  // zero records without bands (the ensemble kernel reads them always)
         d.a.part = m.rec;
         d.a.gpart = nullptr;
         d.planes = h->planes;

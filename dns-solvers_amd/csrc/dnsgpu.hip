@@ -350,6 +350,9 @@ extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
     if (update_wall_words(h)) return fail(DNSGPU_EHIP);
     hipLaunchKernelGGL(column_nbr_kernel, dim3((unsigned)((xy + 255) / 256)), dim3(256), 0, h->stream, g.X, g.Y,
                        h->cnbr);
+    // band records of the all-zero bands (read by the ensemble's kernel)
+    hipLaunchKernelGGL(band_nbr_kernel, dim3((unsigned)((xy + 255) / 256)), dim3(256), 0, h->stream, h->bands, g.X,
+                       g.Y, h->bnbr);
     if (hipGetLastError() != hipSuccess) return fail(set_err(DNSGPU_EHIP, "column records"));
     // the pop buffers start at rest
     h->cur = 0;

@@ -69,23 +69,31 @@ static __global__ __launch_bounds__(kWave* kEnsWaves) void ens_step_kernel(const
     const int nw = gridDim.x * kEnsWaves;
     for (int t = blockIdx.x * kEnsWaves + wave; t < ntiles; t += nw) {
         const EnsTile tl = load_uniform(tiles + t);
-        StepArgs A = load_uniform(&mem[tl.m].a);
-        if (phase) {
-            void* s = const_cast<void*>(A.src);
-            A.src = A.dst;
-            A.dst = s;
-        }
+        // only the fields the column update reads (scalar loads; fewer live SGPRs)
+        const StepArgs* ma = &mem[tl.m].a;
+        StepArgs A;
+        A.g.Z = load_uniform(&ma->g.Z);
+        A.g.nzc = load_uniform(&ma->g.nzc);
+        const void* s0 = load_uniform(&ma->src);
+        void* d0 = load_uniform(&ma->dst);
+        A.src = phase ? d0 : s0;
+        A.dst = phase ? const_cast<void*>(s0) : d0;
+        A.wword = load_uniform(&ma->wword);
+        A.bnbr = load_uniform(&ma->bnbr);
+        A.cnbr = load_uniform(&ma->cnbr);
+        A.omega = load_uniform(&ma->omega);
+        A.G = load_uniform(&ma->G);
+        A.shift = load_uniform(&ma->shift);
+        A.part = load_uniform(&ma->part);
         const int Z = A.g.Z;
         for (int i = lane; i < kNQ * Z; i += kWave) mine[i] = 0.0;
         wave_lds_sync();
         StepAcc acc;
         const double G = *A.G;
         const int cend = tl.col0 + tl.ncol;
-        if (A.bands) {
-            for (int c = tl.col0; c < cend; c++) step_column<T, false, true, true>(A, c, lane, G, A.shift, mine, acc);
-        } else {
-            for (int c = tl.col0; c < cend; c++) step_column<T, false, true, false>(A, c, lane, G, A.shift, mine, acc);
-        }
+        // one code path for every member: a member without solid bands has
+        // all-zero band records, so no chunk takes the band branch
+        for (int c = tl.col0; c < cend; c++) step_column<T, false, true, true>(A, c, lane, G, A.shift, mine, acc);
         wave_lds_sync();
         double* rec = A.part + (long long)tl.lt * ens_record(Z);
         for (int i = lane; i < kNQ * Z; i += kWave) rec[i] = mine[i];
