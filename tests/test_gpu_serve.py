@@ -72,3 +72,23 @@ def test_sequential_single_case_path_runs_the_same_steps():
     res, _ = dnsserve.sweep(dnsserve.requests(items), cfg())
     secs = dnsserve.sequential(dnsserve.requests(items), cfg(), dnsgpu.FP64, [r["steps"] for r in res])
     assert all(s > 0 for s in secs)
+
+
+def test_dnsserve_command_line(tmp_path):
+    """The request-loop command: a short burst of the seeded sweep (three
+    requests, reduced transient/window/cap) prints one JSON object per
+    solve and a summary line."""
+    import json
+    import os
+    import subprocess
+    print("case: dnsserve command", flush=True)
+    exe = os.path.join(os.path.dirname(dnsgpu.LIB_PATH), "..", "bin", "dnsserve")
+    r = subprocess.run([exe, "-b", "-n", "3", "-c", "400", "-t", "100", "-w", "100", "-k", "100", "-p", "fp16"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    lines = [json.loads(x) for x in r.stdout.splitlines()]
+    solves, summary = lines[:-1], lines[-1]
+    assert len(solves) == 3 and summary["summary"] and summary["completed"] == 3
+    for s in solves:
+        assert s["status"] in (0, 1) and 201 <= s["steps"] <= 401 and s["u_tau"] > 0
+    assert "admit" in r.stderr and "done" in r.stderr
