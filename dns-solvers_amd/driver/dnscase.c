@@ -35,6 +35,9 @@ void dns_case_defaults(dns_run_case *c)
     c->init_modes = 4;
     c->riblet_spacing = 16;
     c->riblet_height = 8;
+    c->block_x = 2;
+    c->block_y = 2;
+    c->block_height = 4;
     c->tavg_start = -1;
     c->tavg_every = 0;
     strcpy(c->outdir, ".");
@@ -63,7 +66,7 @@ static int set_key(dns_run_case *c, const char *k, const char *v, char *err, int
     DBL(Rebs) DBL(ub) DBL(CFL) DBL(force) DBL(omega) DBL(init_uc) DBL(init_amp)
     DBL(riblet_spacing) DBL(riblet_height) DBL(riblet_offset)
     INT(xdv) INT(ydv) INT(zdv) INT(TSTR) INT(TEND) INT(period_x) INT(period_y) INT(slip_x) INT(slip_y)
-    INT(offset_x) INT(offset_y) INT(init_modes) INT(tavg_every)
+    INT(offset_x) INT(offset_y) INT(init_modes) INT(tavg_every) INT(block_x) INT(block_y) INT(block_height)
 #undef DBL
 #undef INT
     if (!strcmp(k, "tavg_start")) {
@@ -104,6 +107,7 @@ static int set_key(dns_run_case *c, const char *k, const char *v, char *err, int
         else if (!strcmp(v, "posts")) c->texture = 1;
         else if (!strcmp(v, "stripes")) c->texture = 2;
         else if (!strcmp(v, "riblets")) c->texture = 3;
+        else if (!strcmp(v, "blocks")) c->texture = 4;
         else goto bad;
         return 0;
     }
@@ -173,6 +177,10 @@ int dns_case_read(const char *path, dns_run_case *c, char *err, int errlen)
         snprintf(err, errlen, "riblet spacing must be > 0 and height >= 0");
         return -1;
     }
+    if (c->texture == 4 && (c->block_x < 0 || c->block_y < 0 || c->block_height < 0)) {
+        snprintf(err, errlen, "block sizes and height must be >= 0");
+        return -1;
+    }
     if (c->texture && (c->period_x < 1 || c->period_y < 1)) {
         snprintf(err, errlen, "texture periods must be >= 1");
         return -1;
@@ -201,6 +209,17 @@ void dns_case_masks(const dns_run_case *c, int8_t *bottom, int8_t *top)
 void dns_case_bands(const dns_run_case *c, int16_t *bottom, int16_t *top)
 {
     const int X = c->xdv, Y = c->ydv;
+    if (c->texture == 4) {
+        for (int x = 0; x < X; x++)
+            for (int y = 0; y < Y; y++) {
+                const int in = modp(x - c->offset_x, c->period_x) < c->block_x &&
+                               modp(y - c->offset_y, c->period_y) < c->block_y;
+                const int16_t h = in ? (int16_t)c->block_height : 0;
+                bottom[x * Y + y] = (c->walls == 0 || c->walls == 1) ? h : 0;
+                top[x * Y + y] = (c->walls == 0 || c->walls == 2) ? h : 0;
+            }
+        return;
+    }
     for (int y = 0; y < Y; y++) {
         int16_t hb = 0;
         if (c->texture == 3) {

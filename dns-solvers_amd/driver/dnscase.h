@@ -10,8 +10,10 @@
  *   force      = <double>             body force for constf
  *   precision  = fp64 | fp32 | fp16   (default fp64; fp32/fp16: shifted storage)
  *   omega      = <double>             1/tau; default from Rebs, ub, zdv
- *   texture    = smooth | posts | stripes | riblets  (PSLIP / RSLIP set posts / stripes)
+ *   texture    = smooth | posts | stripes | riblets | blocks  (PSLIP / RSLIP set posts / stripes)
  *   riblet_spacing, riblet_height, riblet_offset   (triangular riblets, cells)
+ *   block_x, block_y, block_height   (blocks: solid posts with side walls, footprint
+ *                                     block_x x block_y every period_x x period_y)
  *   period_x, period_y, slip_x, slip_y, offset_x, offset_y   (texture geometry)
  *   walls      = both | bottom | top  (which walls carry the texture)
  *   init       = rest | poiseuille | turbulent   (for -s; rest is init.c)
@@ -36,7 +38,7 @@ typedef struct dns_run_case {
     double force;
     int precision; /* 0 fp64, 1 fp32, 2 fp16 (shifted storage) */
     double omega;  /* 0 => derived */
-    int texture;   /* 0 smooth, 1 posts, 2 stripes, 3 riblets */
+    int texture;   /* 0 smooth, 1 posts, 2 stripes, 3 riblets, 4 blocks */
     double riblet_spacing, riblet_height, riblet_offset;
     int walls;     /* 0 both, 1 bottom, 2 top */
     int period_x, period_y, slip_x, slip_y, offset_x, offset_y;
@@ -44,6 +46,7 @@ typedef struct dns_run_case {
     double init_uc, init_amp;
     uint64_t init_seed;
     int init_modes;
+    int block_x, block_y, block_height;
     long tavg_start; /* < 0: no time averages */
     int tavg_every;
     char outdir[512];
@@ -67,7 +70,9 @@ void dns_case_masks(const dns_run_case *c, int8_t *bottom, int8_t *top);
 /* Solid bands [xdv][ydv] of triangular riblets (texture 3): the cell
  * (y, z) is solid when its centre z + 1/2 lies below the triangle of
  * spanwise period riblet_spacing and tip height riblet_height at y + 1/2.
- * Streamwise-invariant.  Zero for every other texture. */
+ * Streamwise-invariant.  Blocks (texture 4): block_height solid rows over
+ * the footprint (x - offset_x) mod period_x < block_x and
+ * (y - offset_y) mod period_y < block_y.  Zero for every other texture. */
 void dns_case_bands(const dns_run_case *c, int16_t *bottom, int16_t *top);
 
 #endif

@@ -228,6 +228,9 @@ static void write_report(const dns_run_case *c, double omega, double seconds, do
     if (c->texture == 3)
         fprintf(tm, "texture: riblets spacing %g height %g\n", c->riblet_spacing, c->riblet_height);
     if (c->texture == 1) fprintf(tm, "slip option: PSLIP\n");
+    if (c->texture == 4)
+        fprintf(tm, "texture: blocks %dx%d height %d every %dx%d\n", c->block_x, c->block_y, c->block_height,
+                c->period_x, c->period_y);
     fprintf(tm, "Total computational time is: %f s\n", seconds);
     fprintf(tm, "Unit non-dimensional time is: %f iterations\n", utime);
     fprintf(tm, "Starting time: %d\n", c->TSTR);
@@ -290,7 +293,7 @@ int main(int argc, char *argv[])
     API(dnsgpu_set_walls(h, mb, mt));
     free(mb);
     free(mt);
-    if (c.texture == 3) {
+    if (c.texture == 3 || c.texture == 4) {
         int16_t *hb = malloc(sizeof(int16_t) * (size_t)c.xdv * c.ydv);
         int16_t *ht = malloc(sizeof(int16_t) * (size_t)c.xdv * c.ydv);
         if (!hb || !ht) return 1;

@@ -134,6 +134,15 @@ def riblet_bands(X, Y, spacing, height, offset=0.0):
     return np.ascontiguousarray(np.broadcast_to(hb[None, :], (X, Y)))
 
 
+def block_bands(X, Y, period_x, period_y, block_x, block_y, height, offset_x=0, offset_y=0):
+    """Solid posts with side walls (blocks): `height` solid rows over the
+    footprint (x - offset_x) mod period_x < block_x and (y - offset_y) mod
+    period_y < block_y.  Same rule as dns_case_bands (texture blocks)."""
+    x = (np.arange(X) - offset_x) % period_x < block_x
+    y = (np.arange(Y) - offset_y) % period_y < block_y
+    return np.ascontiguousarray(np.where(x[:, None] & y[None, :], height, 0).astype(np.int16))
+
+
 def band_mask(X, Y, Z, hb, ht, bottom=None, top=None):
     """Reference node codes [X][Y][Z+2] with solid bands as -1 nodes."""
     s = np.zeros((X, Y, Z + 2), dtype=np.int8)

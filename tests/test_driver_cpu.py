@@ -30,7 +30,8 @@ class RunCase(ctypes.Structure):
                 ("period_x", ctypes.c_int), ("period_y", ctypes.c_int), ("slip_x", ctypes.c_int),
                 ("slip_y", ctypes.c_int), ("offset_x", ctypes.c_int), ("offset_y", ctypes.c_int),
                 ("init", ctypes.c_int), ("init_uc", ctypes.c_double), ("init_amp", ctypes.c_double),
-                ("init_seed", ctypes.c_uint64), ("init_modes", ctypes.c_int), ("tavg_start", ctypes.c_long),
+                ("init_seed", ctypes.c_uint64), ("init_modes", ctypes.c_int), ("block_x", ctypes.c_int),
+                ("block_y", ctypes.c_int), ("block_height", ctypes.c_int), ("tavg_start", ctypes.c_long),
                 ("tavg_every", ctypes.c_int), ("outdir", ctypes.c_char * 512)]
 
 
@@ -141,3 +142,20 @@ def test_riblet_voxeliser_matches_test_replica(lib, spacing, height, offset):
     assert np.array_equal(b, ref) and np.array_equal(t, ref)
     # cell centres sample the triangle: the tallest column is within one cell of the tip
     assert height - 1 <= b.max() <= height
+
+
+def test_block_voxeliser_matches_test_replica(lib):
+    c = RunCase()
+    lib.dns_case_defaults(ctypes.byref(c))
+    c.xdv, c.ydv, c.zdv = 12, 16, 40
+    c.texture = 4
+    c.period_x, c.period_y, c.offset_x, c.offset_y = 6, 8, 1, 3
+    c.block_x, c.block_y, c.block_height = 2, 3, 5
+    c.walls = 1  # bottom wall only
+    b = np.zeros((12, 16), np.int16)
+    t = np.zeros((12, 16), np.int16)
+    lib.dns_case_bands(ctypes.byref(c), b.ctypes.data, t.ctypes.data)
+    assert np.array_equal(b, fields.block_bands(12, 16, 6, 8, 2, 3, 5, 1, 3))
+    assert not t.any()
+    # footprint area: 2 x 3 cells in each of the 2 x 2 periods
+    assert (b == 5).sum() == 2 * 3 * 2 * 2 and set(np.unique(b)) == {0, 5}

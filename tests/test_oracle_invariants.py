@@ -216,3 +216,24 @@ def test_riblets_increase_drag_over_flat_wall_at_fixed_flow():
     la = o_flat.run(a, fa, 2000)
     lb = o_rib.run(b, fb, 2000)
     assert lb["force"][-1] > 1.2 * la["force"][-1]
+
+
+@pytest.mark.parametrize("blocks", [(4, 4, 2, 2, 3), (8, 8, 3, 5, 6)])
+def test_block_streaming_exactly_once_and_mass(blocks):
+    # posts with side walls (height fields from both walls): every fluid
+    # slot written exactly once, mass conserved, solid slots untouched
+    X, Y, Z = 8, 16, 20
+    hb = fields.block_bands(X, Y, *blocks)
+    ht = fields.block_bands(X, Y, *blocks, offset_x=1, offset_y=2)
+    s = fields.band_mask(X, Y, Z, hb, ht)
+    o = pyoracle.Oracle(X, Y, Z, 1.6, 0.05, 0.25, forcing=1, F=1e-5, mask=s)
+    cnt = o.coverage()
+    fluid = s == 0
+    assert (cnt[fluid] == 1).all()
+    assert (cnt[~fluid] == 0).all()
+    ft = fields.poiseuille_field(X, Y, Z, 0.05, noise=0.1, seed=9, solid=~fluid)
+    f = ft.copy()
+    m0 = ft.sum()
+    o.run(ft, f, 30)
+    assert abs(ft.sum() - m0) < 1e-13 * m0
+    assert (ft[~fluid] == 0).all()
