@@ -124,9 +124,14 @@ __device__ __forceinline__ void collide_fp64(double (&f)[Q], double ueq0, double
 // The same update on shifted populations g = f - w in FP32 (FMA allowed):
 //   drho = sum g, rho = 1 + drho, j = sum e g, u = j/rho
 //   g' = g + Fi - omega*(g - (feq - w)),  feq - w = w*(drho + rho*(3eu + 4.5eu^2 - 1.5u^2))
+// S: the populations are held multiplied by S (a power of two, so every
+// operation below rounds exactly as without it): S = 1 for FP32 storage,
+// 2^16 for the FP16 storage's scaled values, which then need no rescaling.
+template <int LOG2S = 0>
 __device__ __forceinline__ void collide_fp32(float (&g)[Q], float omega, float Gx, float& drho, float& u0,
                                              float& u1, float& u2, float& jx)
 {
+    constexpr float S = (float)(1 << LOG2S), Sinv = 1.0f / (float)(1 << LOG2S);
     float sx = 0.f, sy = 0.f, sz = 0.f, sr = 0.f;
     static_for<0, Q>([&](auto ac) {
         constexpr int a = decltype(ac)::value;
@@ -138,6 +143,12 @@ __device__ __forceinline__ void collide_fp32(float (&g)[Q], float omega, float G
         if constexpr (EZ[a] == -1) sz -= g[a];
         sr += g[a];
     });
+    if constexpr (LOG2S != 0) {
+        sx *= Sinv;
+        sy *= Sinv;
+        sz *= Sinv;
+        sr *= Sinv;
+    }
     drho = sr;
     jx = sx;
     const float rho = 1.0f + sr;
@@ -149,7 +160,7 @@ __device__ __forceinline__ void collide_fp32(float (&g)[Q], float omega, float G
     const float om1 = 1.0f - omega;
     static_for<0, Q>([&](auto ac) {
         constexpr int a = decltype(ac)::value;
-        constexpr float w = (float)WD[a];
+        constexpr float w = (float)WD[a] * S;
         const float eu = edotf<a>(u0, u1, u2);
         // feq - w
         const float poly = __builtin_fmaf(eu, __builtin_fmaf(4.5f, eu, 3.0f), -v15);

@@ -205,18 +205,29 @@ extern "C" int dnsens_add(dnsens* e, dnsgpu* h)
     }
     // a state fresh from a load or an init holds post-stream populations:
     // its first step (local gather) runs through the single-case kernels
+    DevState st;
+    HIPCHK(hipMemcpyAsync(&st, h->st, sizeof st, hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    double stress = 0.0;
     if (h->poststream) {
-        DevState st;
-        HIPCHK(hipMemcpyAsync(&st, h->st, sizeof st, hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipStreamSynchronize(h->stream));
         int rc = do_step(h, true, h->ring, (long long)st.steps);
         if (rc) {
             free_member(m);
             return rc;
         }
+        LogRow row;
+        HIPCHK(hipMemcpyAsync(&st, h->st, sizeof st, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipMemcpyAsync(&row, h->ring, sizeof row, hipMemcpyDeviceToHost, h->stream));
         HIPCHK(hipStreamSynchronize(h->stream));
+        stress = row.stress;
     }
-    m.info.finite = 1;
+    // the member's state as of now (dnsens_info before its first dnsens_step)
+    m.info.steps = st.steps;
+    m.info.n_rows = st.n_rows;
+    m.info.mean_pgrad = st.n_rows > 0 ? st.s_pgrad / st.n_rows : 0.0;
+    m.info.force = st.G;
+    m.info.last_stress = stress;
+    m.info.finite = std::isfinite(st.G) && std::isfinite(stress) && std::isfinite(st.s_pgrad);
     h->owner = e;
     e->mem.push_back(m);
     e->used += need;

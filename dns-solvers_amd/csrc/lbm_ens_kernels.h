@@ -89,7 +89,7 @@ static __global__ __launch_bounds__(kWave* kEnsWaves) void ens_step_kernel(const
         for (int i = lane; i < kNQ * Z; i += kWave) mine[i] = 0.0;
         wave_lds_sync();
         StepAcc acc;
-        const double G = *A.G;
+        const double G = load_uniform(A.G);
         const int cend = tl.col0 + tl.ncol;
         // one code path for every member: a member without solid bands has
         // all-zero band records, so no chunk takes the band branch

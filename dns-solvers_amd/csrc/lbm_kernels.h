@@ -177,6 +177,13 @@ __device__ __forceinline__ uint32_t load_uniform_u32(const uint32_t* p)
     return *(const __attribute__((address_space(4))) uint32_t*)p;
 }
 
+// Bytes of global memory (HBM).  The field pointers are held in this
+// address space: a pointer loaded from a table (the ensemble's member
+// records) is otherwise generic, and every access through a generic pointer
+// is a flat instruction with a per-lane 64-bit address.
+typedef __attribute__((address_space(1))) char gbyte;
+#define GLOBAL_AS __attribute__((address_space(1)))
+
 // Wave-uniform load through the constant address space (scalar loads) of
 // a table entry that no kernel running at the same time writes.
 template <typename S>
@@ -288,33 +295,33 @@ __device__ __forceinline__ void load_local19(const T* __restrict__ src, const Co
 // cache line is fetched) and are then overwritten by the wall rule: uniform
 // decisions from the column's wall word, one active lane.
 template <typename T, bool FIRST>
-__device__ __forceinline__ void gather_wave(const char* const (&sp)[3][3], int Z, uint32_t zoff, uint32_t zoff_up,
+__device__ __forceinline__ void gather_wave(const gbyte* const (&sp)[3][3], int zb, uint32_t zoff, uint32_t zoff_up,
                                             uint32_t zoff_dn, bool bot_lane, bool top_lane, uint32_t ww,
                                             T (&v)[Q])
 {
     // sp[ex + 1][ey + 1]: byte address of the block of column (x - ex, y - ey)
-    const int zb = Z * (int)sizeof(T);  // bytes of one direction segment
     static_for<0, Q>([&](auto ac) {
         constexpr int a = decltype(ac)::value;
-        const char* p;
+        const gbyte* p;
         uint32_t off;
         if constexpr (FIRST) {
             p = sp[1][1] + a * zb;
             off = zoff;
         } else {
-            p = sp[EX[a] + 1][EY[a] + 1] + (a * zb - EZ[a] * (int)sizeof(T));
+            p = sp[EX[a] + 1][EY[a] + 1] + a * zb;
             off = EZ[a] == 1 ? zoff_up : (EZ[a] == -1 ? zoff_dn : zoff);
         }
-        v[a] = *(const T*)(p + off);
+        v[a] = *(const GLOBAL_AS T*)(p + off);
     });
     if constexpr (!FIRST) {
         if (bot_lane) {
             static_for<0, Q>([&](auto ac) {
                 constexpr int a = decltype(ac)::value;
                 if constexpr (EZ[a] == 1) {
+                    // row 0: a wave-uniform address
                     const bool spec = (ww >> wall_bit<a>()) & 1u;
-                    const char* p = spec ? sp[EX[a] + 1][EY[a] + 1] + MIR[a] * zb : sp[1][1] + OPP[a] * zb;
-                    v[a] = *(const T*)(p + zoff);
+                    const gbyte* p = spec ? sp[EX[a] + 1][EY[a] + 1] + MIR[a] * zb : sp[1][1] + OPP[a] * zb;
+                    v[a] = *(const GLOBAL_AS T*)p;
                 }
             });
         }
@@ -322,9 +329,10 @@ __device__ __forceinline__ void gather_wave(const char* const (&sp)[3][3], int Z
             static_for<0, Q>([&](auto ac) {
                 constexpr int a = decltype(ac)::value;
                 if constexpr (EZ[a] == -1) {
+                    // row Z - 1, the segment's last: a wave-uniform address
                     const bool spec = (ww >> wall_bit<a>()) & 1u;
-                    const char* p = spec ? sp[EX[a] + 1][EY[a] + 1] + MIR[a] * zb : sp[1][1] + OPP[a] * zb;
-                    v[a] = *(const T*)(p + zoff);
+                    const gbyte* p = spec ? sp[EX[a] + 1][EY[a] + 1] + MIR[a] * zb : sp[1][1] + OPP[a] * zb;
+                    v[a] = *(const GLOBAL_AS T*)(p + (zb - (int)sizeof(T)));
                 }
             });
         }
@@ -358,12 +366,14 @@ __device__ __forceinline__ void collide_stored(float (&v)[Q], float omega, float
 __device__ __forceinline__ void collide_stored(_Float16 (&v)[Q], float omega, float G, float& drho, float& u0,
                                                float& u1, float& u2, float& jx)
 {
+    // collide the stored values g 2^16 directly (bitwise the result of
+    // unscaling, colliding and scaling back)
     float g[Q];
 #pragma unroll
-    for (int a = 0; a < Q; a++) g[a] = Store<_Float16>::to_g(v[a]);
-    collide_fp32(g, omega, G, drho, u0, u1, u2, jx);
+    for (int a = 0; a < Q; a++) g[a] = (float)v[a];
+    collide_fp32<16>(g, omega, G, drho, u0, u1, u2, jx);
 #pragma unroll
-    for (int a = 0; a < Q; a++) v[a] = Store<_Float16>::from_g(g[a]);
+    for (int a = 0; a < Q; a++) v[a] = (_Float16)g[a];
 }
 
 // Register accumulators of one wave over the columns it updates.
@@ -389,12 +399,12 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
 {
     const int Z = A.g.Z, nzc = A.g.nzc;
     const int ctop = (Z - 1) / kWave, ltop = (Z - 1) % kWave;
-    const T* __restrict__ src = (const T*)A.src;
-    char* __restrict__ d8 = (char*)A.dst;
+    const gbyte* __restrict__ s8 = (const gbyte*)A.src;
+    gbyte* __restrict__ d8 = (gbyte*)A.dst;
     // byte addresses of the 3x3 neighbourhood's column blocks in the source
     // and the destination, [ex + 1][ey + 1] = column (x - ex, y - ey)
-    const char* sp[3][3];
-    char* dp[3][3];
+    const gbyte* sp[3][3];
+    gbyte* dp[3][3];
     {
         const ColNbr cr = load_uniform(reinterpret_cast<const ColNbr*>(A.cnbr) + col);
         const uint32_t qzb = (uint32_t)(Q * Z * (int)sizeof(T));  // bytes of a column block (< 2^17)
@@ -403,7 +413,7 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
 #pragma unroll
             for (int j = 0; j < 3; j++) {
                 const unsigned long long o = (unsigned long long)cr.w[3 * i + j] * qzb;
-                sp[i][j] = (const char*)src + o;
+                sp[i][j] = s8 + o;
                 dp[i][j] = d8 + o;
             }
     }
@@ -427,8 +437,15 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
         const bool bot_lane = ch == 0 && lane == 0;
         const bool top_lane = ch == ctop && lane == ltop;
         const uint32_t zoff = (uint32_t)z * (uint32_t)sizeof(T);
-        const uint32_t zoff_up = (uint32_t)(z == 0 ? 1 : z) * (uint32_t)sizeof(T);
-        const uint32_t zoff_dn = (uint32_t)(z == Z - 1 ? Z - 2 : z) * (uint32_t)sizeof(T);
+        // upstream rows of the e_z = +1 and -1 populations (z - 1, z + 1),
+        // clamped into the column at the walls
+        const uint32_t zoff_up = (uint32_t)(z == 0 ? 0 : z - 1) * (uint32_t)sizeof(T);
+        const uint32_t zoff_dn = (uint32_t)(z == Z - 1 ? Z - 1 : z + 1) * (uint32_t)sizeof(T);
+        // zb (bytes of a direction segment) is made opaque per chunk, so the
+        // 19 + 19 per-direction addresses are formed in each chunk from the
+        // nine column bases instead of being held (and spilled) across it
+        int zbc = zb;
+        asm volatile("" : "+s"(zbc));
         T v[Q];
         // chunk touches a band of the neighbourhood (upstream rows z-1..z+1)
         bool slow = false, solid_own = false;
@@ -440,7 +457,7 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
         // a node next to a solid band pulls, like any other, from its
         // neighbour: the solid node's slot holds the bounced-back value
         // (stored there by this node at the previous step, below)
-        gather_wave<T, FIRST>(sp, Z, zoff, zoff_up, zoff_dn, bot_lane, top_lane, ww, v);
+        gather_wave<T, FIRST>(sp, zbc, zoff, zoff_up, zoff_dn, bot_lane, top_lane, ww, v);
 
         double mass_node, jx_raw, g11, g12, g13, g14;
         double gx[Q];  // post-collision values of the e_x != 0 directions (budget)
@@ -523,14 +540,18 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
                 const bool inside = zt >= 0 && zt < Z;
                 // x + e_a is not a fluid node (a band node or a wall row)
                 const bool nb_solid = a == 0 ? solid_own : (!inside || band_solid(bw[1 - EX[a]][1 - EY[a]], zt, Z));
-                if (!solid_own || nb_solid) *(T*)(dp[1][1] + a * zb + zoff) = v[a];
+                if (!solid_own || nb_solid) *(GLOBAL_AS T*)(dp[1][1] + a * zbc + zoff) = v[a];
                 if (a != 0 && !solid_own && nb_solid && inside)
-                    *(T*)(dp[1 - EX[a]][1 - EY[a]] + (OPP[a] * zb + EZ[a] * (int)sizeof(T)) + zoff) = v[a];
+                    *(GLOBAL_AS T*)(dp[1 - EX[a]][1 - EY[a]] + (OPP[a] * zbc + EZ[a] * (int)sizeof(T)) + zoff) = v[a];
             });
         } else {
+            // the row offset as a 32-bit value of this block, so the stores
+            // take the (SGPR base + 32-bit VGPR offset) form
+            uint32_t zo = zoff;
+            asm volatile("" : "+v"(zo));
             static_for<0, Q>([&](auto ac) {
                 constexpr int a = decltype(ac)::value;
-                *(T*)(dp[1][1] + a * zb + zoff) = v[a];
+                *(GLOBAL_AS T*)(dp[1][1] + a * zbc + zo) = v[a];
             });
         }
     }

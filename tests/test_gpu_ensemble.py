@@ -189,6 +189,30 @@ def test_admission_by_footprint_and_member_rules():
     ens.close()
 
 
+def test_info_is_current_from_admission():
+    """dnsens_info right after dnsens_add reports the member's counter and
+    force as they are then: a fresh state has taken its first step (counter
+    1), a state that stepped alone keeps its counter; after a dnsens_step
+    the counters advance by the steps taken."""
+    print("case: info at admission", flush=True)
+    ens = dnsgpu.Ensemble(precision=dnsgpu.FP64)
+    fresh = make((16, 8, 10), "smooth")
+    stepped = make((16, 8, 10), "smooth", seed=3)
+    stepped.step(6)
+    g = stepped.force()
+    ens.add(fresh)
+    ens.add(stepped)
+    a, b = ens.info(fresh), ens.info(stepped)
+    assert a["steps"] == 1 and b["steps"] == 6
+    assert a["finite"] == 1 and b["finite"] == 1
+    assert b["force"] == g and a["force"] != 0.0
+    ens.step(4)
+    assert ens.info(fresh)["steps"] == 5 and ens.info(stepped)["steps"] == 10
+    for ch in (fresh, stepped):
+        ch.close()
+    ens.close()
+
+
 def test_nan_member_is_flagged_and_others_unaffected():
     print("case: NaN member", flush=True)
     ens = dnsgpu.Ensemble(precision=dnsgpu.FP64)
