@@ -290,15 +290,16 @@ static int upload_tables(dnsens* e)
         d.chunk0 = chunk;
         d.nchunks = (ens_record(h->g.Z) + kWave - 1) / kWave;
         chunk += d.nchunks;
-        const int cpt = ens_tile_cols(h->g.ncols);
         FinalizeArgs F = fin_args(h, true, h->ring, 0, 0);
         F.cntr_dev = 1;
         e->h_fin[i] = F;
         for (int t = 0; t < m.ntiles; t++) {
-            EnsTile tl;
+            EnsTile tl{};
             tl.m = i;
-            tl.col0 = t * cpt;
-            tl.ncol = std::min(cpt, h->g.ncols - tl.col0);
+            // columns t, t + S, ... below ncols: at most ens_tile_cols of them
+            tl.col0 = t;
+            tl.stride = m.ntiles;
+            tl.ncol = (h->g.ncols - t + m.ntiles - 1) / m.ntiles;
             tl.lt = t;
             e->h_tiles.push_back(tl);
         }

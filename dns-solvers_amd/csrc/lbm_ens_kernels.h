@@ -3,8 +3,11 @@
 // numbers advance together, one launch of each kernel per step for all of
 // them.
 //
-// Work is described by a tile table: a tile is a few consecutive columns
-// of one member, updated by one wave.  The step kernel's grid is sized to
+// Work is described by a tile table: a tile is a few columns of one
+// member, updated by one wave.  Tile t of a member with S tiles holds the
+// columns t, t + S, t + 2S, ...: the waves that run at the same time then
+// update runs of adjacent columns, as the single-case kernel's do (about 2%
+// faster than consecutive columns per tile).  The step kernel's grid is sized to
 // what the device holds at once and its waves loop over the table with no
 // block-level synchronisation; each tile writes one contiguous record of
 // its per-plane and global sums, so every member's reductions run in an
@@ -43,6 +46,8 @@ struct EnsTile {
     int col0;  // first column
     int ncol;  // columns
     int lt;    // tile index within the member (record slot)
+    int stride;  // column stride (the member's tile count)
+    int pad[3];
 };
 
 // Orders the wave's LDS accesses across lanes (the zeroing and record
@@ -90,10 +95,10 @@ static __global__ __launch_bounds__(kWave* kEnsWaves) void ens_step_kernel(const
         wave_lds_sync();
         StepAcc acc;
         const double G = load_uniform(A.G);
-        const int cend = tl.col0 + tl.ncol;
         // one code path for every member: a member without solid bands has
         // all-zero band records, so no chunk takes the band branch
-        for (int c = tl.col0; c < cend; c++) step_column<T, false, true, true>(A, c, lane, G, A.shift, mine, acc);
+        for (int k = 0; k < tl.ncol; k++)
+            step_column<T, false, true, true>(A, tl.col0 + k * tl.stride, lane, G, A.shift, mine, acc);
         wave_lds_sync();
         double* rec = A.part + (long long)tl.lt * ens_record(Z);
         for (int i = lane; i < kNQ * Z; i += kWave) rec[i] = mine[i];
