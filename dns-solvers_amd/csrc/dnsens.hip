@@ -62,6 +62,7 @@ struct dnsens {
     std::vector<FinalizeArgs> h_fin;
     std::vector<EnsTile> h_tiles;
     int ntiles, nchunks, zmax;
+    bool shl_lds;  // the step kernel keeps the members' shifts in LDS
     size_t lds;
     int resident;       // step-kernel blocks resident at once for this lds
     int blocks_per_cu;  // grid = blocks_per_cu * ncu (capped)
@@ -175,7 +176,7 @@ extern "C" size_t dnsens_member_bytes(const dnsgpu* h)
 extern "C" size_t dnsens_footprint(const dns_case* c)
 {
     const size_t fp = dnsgpu_footprint(c);
-    if (!fp) return 0;
+    if (!fp || c->zdv > kEnsZmax) return 0;
     const int ncols = c->xdv * c->ydv, cpt = ens_tile_cols(ncols);
     return fp + sizeof(double) * (size_t)((ncols + cpt - 1) / cpt) * ens_record(c->zdv);
 }
@@ -189,6 +190,9 @@ extern "C" int dnsens_add(dnsens* e, dnsgpu* h)
     if (find_member(e, h) >= 0) return set_err(DNSGPU_EARG, "handle is already a member");
     if ((int)e->mem.size() >= DNSENS_MAX_MEMBERS)
         return set_err(DNSGPU_EARG, "ensemble holds %d members already", DNSENS_MAX_MEMBERS);
+    if (h->g.Z > kEnsZmax)
+        return set_err(DNSGPU_EARG, "member zdv %d above the ensemble's %d (per-wave plane sums in LDS)", h->g.Z,
+                       kEnsZmax);
     const size_t need = dnsens_member_bytes(h);
     if (e->used + need > e->budget)
         return set_err(DNSGPU_ENOMEM, "member needs %zu bytes, %zu of %zu in use", need, e->used, e->budget);
@@ -256,9 +260,12 @@ extern "C" size_t dnsens_bytes_used(const dnsens* e) { return e ? e->used : 0; }
 extern "C" size_t dnsens_budget(const dnsens* e) { return e ? e->budget : 0; }
 
 template <typename T>
-static int step_occupancy(size_t lds, int* per_cu)
+static int step_occupancy(bool shl, size_t lds, int* per_cu)
 {
-    HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, ens_step_kernel<T>, kWave * kEnsWaves, lds));
+    if (shl)
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, ens_step_kernel<T, true>, kWave * kEnsWaves, lds));
+    else
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, ens_step_kernel<T, false>, kWave * kEnsWaves, lds));
     return 0;
 }
 
@@ -344,9 +351,11 @@ static int upload_tables(dnsens* e)
     }
     if (zmax != e->zmax || e->resident == 0) {
         e->zmax = zmax;
-        e->lds = ens_lds_bytes(std::max(zmax, 1));
+        e->shl_lds = ens_lds_bytes(std::max(zmax, 1), true) <= kEnsLdsMax;
+        e->lds = ens_lds_bytes(std::max(zmax, 1), e->shl_lds);
         int per = 0;
-        int rc = with_storage(e->precision, [&](auto t) { return step_occupancy<decltype(t)>(e->lds, &per); });
+        int rc = with_storage(e->precision,
+                              [&](auto t) { return step_occupancy<decltype(t)>(e->shl_lds, e->lds, &per); });
         if (rc) return rc;
         e->resident = std::max(1, per) * e->ncu;
         e->layout_changed = true;
@@ -367,8 +376,12 @@ static int upload_tables(dnsens* e)
 template <typename T>
 static void enqueue_step_only(dnsens* e, int phase)
 {
-    hipLaunchKernelGGL(ens_step_kernel<T>, dim3(e->grid), dim3(kWave * kEnsWaves), e->lds, e->stream, e->d_mem,
-                       e->d_tiles, e->ntiles, std::max(e->zmax, 1), phase);
+    if (e->shl_lds)
+        hipLaunchKernelGGL((ens_step_kernel<T, true>), dim3(e->grid), dim3(kWave * kEnsWaves), e->lds, e->stream,
+                           e->d_mem, e->d_tiles, e->ntiles, std::max(e->zmax, 1), phase);
+    else
+        hipLaunchKernelGGL((ens_step_kernel<T, false>), dim3(e->grid), dim3(kWave * kEnsWaves), e->lds, e->stream,
+                           e->d_mem, e->d_tiles, e->ntiles, std::max(e->zmax, 1), phase);
 }
 
 // k steps of every member: step, reduction, finisher per step; log row of

@@ -83,6 +83,30 @@ def test_members_match_single_case_path(forcing):
     ens.close()
 
 
+def test_tall_member_matches_single_case_path():
+    """A member taller than 292 rows makes the step kernel read the plane
+    shifts from global memory instead of LDS: the same bitwise results
+    against the single-case path (constant F).  A case above 512 rows has
+    no footprint (neither path takes it)."""
+    print("case: tall member", flush=True)
+    n, F = 12, 2e-5
+    ens = dnsgpu.Ensemble(precision=dnsgpu.FP64)
+    shapes = [(8, 8, 300), (16, 8, 10)]
+    members = [make(s, "smooth", dnsgpu.CONST_F, F) for s in shapes]
+    for ch in members:
+        ens.add(ch)
+    ens.step(n)
+    for shape, ch in zip(shapes, members):
+        ref = make(shape, "smooth", dnsgpu.CONST_F, F)
+        ref.step(n + 1, cntr0=0)
+        assert np.array_equal(ch.save_aos()[0], ref.save_aos()[0]), shape
+        ref.close()
+    assert dnsgpu.member_footprint(8, 8, 513, dnsgpu.FP64) == 0
+    for ch in members:
+        ch.close()
+    ens.close()
+
+
 @pytest.mark.parametrize("precision", [dnsgpu.FP64, dnsgpu.FP32, dnsgpu.FP16])
 def test_alone_equals_in_mix(precision):
     """A member's result does not depend on the other members: the same
