@@ -53,7 +53,7 @@ __device__ __forceinline__ double edot(double u0, double u1, double u2)
 template <int a>
 __device__ __forceinline__ float edotf(float u0, float u1, float u2)
 {
-    float s = 0.0f;
+    float s = -0.0f;  // the additive identity: the first term costs no add
     if constexpr (EX[a] != 0) s += (EX[a] > 0) ? u0 : -u0;
     if constexpr (EY[a] != 0) s += (EY[a] > 0) ? u1 : -u1;
     if constexpr (EZ[a] != 0) s += (EZ[a] > 0) ? u2 : -u2;
@@ -124,24 +124,37 @@ __device__ __forceinline__ void collide_fp64(double (&f)[Q], double ueq0, double
 // The same update on shifted populations g = f - w in FP32 (FMA allowed):
 //   drho = sum g, rho = 1 + drho, j = sum e g, u = j/rho
 //   g' = g + Fi - omega*(g - (feq - w)),  feq - w = w*(drho + rho*(3eu + 4.5eu^2 - 1.5u^2))
-// S: the populations are held multiplied by S (a power of two, so every
-// operation below rounds exactly as without it): S = 1 for FP32 storage,
-// 2^16 for the FP16 storage's scaled values, which then need no rescaling.
+// evaluated a pair of opposite directions (a, b = opp(a)) at a time, which
+// share w, |e.u| and e_x u_x:
+//   moments from g_a + g_b and g_a - g_b;
+//   q = 4.5eu^2 - 1.5u^2, poly_a,b = q +- 3eu;
+//   omega (feq - w) = K1 poly + K2 with K1 = omega w rho, K2 = omega w drho;
+//   Fi_a,b = t +- 3 e_x w G, t = w G (9 e_x eu - 3 u_x)  (Fi = w G (-3 u_x) when e_x = 0);
+//   g'_a = (1 - omega) g_a + K1 poly_a + K2 + Fi_a.
+// S: the populations are held multiplied by S (a power of two): S = 1 for
+// FP32 storage, 2^16 for the FP16 storage's scaled values; the moments are
+// unscaled once and the weights carry S, so no per-population rescaling.
 template <int LOG2S = 0>
 __device__ __forceinline__ void collide_fp32(float (&g)[Q], float omega, float Gx, float& drho, float& u0,
                                              float& u1, float& u2, float& jx)
 {
     constexpr float S = (float)(1 << LOG2S), Sinv = 1.0f / (float)(1 << LOG2S);
-    float sx = 0.f, sy = 0.f, sz = 0.f, sr = 0.f;
-    static_for<0, Q>([&](auto ac) {
+    // -0 is the additive identity (x + -0 = x for every x), so the first
+    // term of each sum costs no add
+    float sx = -0.0f, sy = -0.0f, sz = -0.0f, sr = g[0];
+    static_for<1, Q>([&](auto ac) {
         constexpr int a = decltype(ac)::value;
-        if constexpr (EX[a] == 1) sx += g[a];
-        if constexpr (EX[a] == -1) sx -= g[a];
-        if constexpr (EY[a] == 1) sy += g[a];
-        if constexpr (EY[a] == -1) sy -= g[a];
-        if constexpr (EZ[a] == 1) sz += g[a];
-        if constexpr (EZ[a] == -1) sz -= g[a];
-        sr += g[a];
+        if constexpr (a < OPP[a]) {
+            constexpr int b = OPP[a];
+            const float pab = g[a] + g[b], mab = g[a] - g[b];
+            sr += pab;
+            if constexpr (EX[a] == 1) sx += mab;
+            if constexpr (EX[a] == -1) sx -= mab;
+            if constexpr (EY[a] == 1) sy += mab;
+            if constexpr (EY[a] == -1) sy -= mab;
+            if constexpr (EZ[a] == 1) sz += mab;
+            if constexpr (EZ[a] == -1) sz -= mab;
+        }
     });
     if constexpr (LOG2S != 0) {
         sx *= Sinv;
@@ -158,23 +171,42 @@ __device__ __forceinline__ void collide_fp32(float (&g)[Q], float omega, float G
     u2 = sz * rinv;
     const float v15 = 1.5f * __builtin_fmaf(u0, u0, __builtin_fmaf(u1, u1, u2 * u2));
     const float om1 = 1.0f - omega;
-    static_for<0, Q>([&](auto ac) {
+    const float m3u0 = -3.0f * u0;
+    // per weight class k (rest 1/3, axis 1/18, diagonal 1/36, times S)
+    constexpr float Wk[3] = {(float)(1. / 3.) * S, (float)(1. / 18.) * S, (float)(1. / 36.) * S};
+    float K1[3], K2C[3], K2p[3], K2m[3], Ck[3], Pk[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+        const float ow = omega * Wk[k], wG = Wk[k] * Gx;
+        K1[k] = ow * rho;
+        const float K2 = ow * sr;
+        Ck[k] = wG * m3u0;
+        K2C[k] = K2 + Ck[k];
+        K2p[k] = __builtin_fmaf(3.0f, wG, K2);
+        K2m[k] = __builtin_fmaf(-3.0f, wG, K2);
+        Pk[k] = 9.0f * wG;
+    }
+    g[0] = __builtin_fmaf(om1, g[0], __builtin_fmaf(K1[0], -v15, K2C[0]));
+    static_for<1, Q>([&](auto ac) {
         constexpr int a = decltype(ac)::value;
-        constexpr float w = (float)WD[a] * S;
-        const float eu = edotf<a>(u0, u1, u2);
-        // feq - w
-        const float poly = __builtin_fmaf(eu, __builtin_fmaf(4.5f, eu, 3.0f), -v15);
-        const float feqs = w * __builtin_fmaf(rho, poly, sr);
-        float br;
-        if constexpr (EX[a] == 0)
-            br = -3.0f * u0;
-        else if constexpr (EX[a] == 1)
-            br = __builtin_fmaf(9.0f, eu, 3.0f * (1.0f - u0));
-        else
-            br = __builtin_fmaf(-9.0f, eu, 3.0f * (-1.0f - u0));
-        const float Fi = (w * Gx) * br;
-        // g + Fi - omega*(g - feqs) = (1-omega) g + omega feqs + Fi
-        g[a] = __builtin_fmaf(om1, g[a], __builtin_fmaf(omega, feqs, Fi));
+        if constexpr (a < OPP[a]) {
+            constexpr int b = OPP[a];
+            constexpr int k = (EX[a] != 0) + (EY[a] != 0) + (EZ[a] != 0);  // 1 axis, 2 diagonal
+            const float eu = edotf<a>(u0, u1, u2);
+            const float q = __builtin_fmaf(4.5f * eu, eu, -v15);
+            const float pa = __builtin_fmaf(3.0f, eu, q), pb = __builtin_fmaf(-3.0f, eu, q);
+            float ia, ib;
+            if constexpr (EX[a] == 0) {
+                ia = __builtin_fmaf(K1[k], pa, K2C[k]);
+                ib = __builtin_fmaf(K1[k], pb, K2C[k]);
+            } else {
+                const float t = __builtin_fmaf(EX[a] > 0 ? Pk[k] : -Pk[k], eu, Ck[k]);
+                ia = __builtin_fmaf(K1[k], pa, (EX[a] > 0 ? K2p[k] : K2m[k]) + t);
+                ib = __builtin_fmaf(K1[k], pb, (EX[a] > 0 ? K2m[k] : K2p[k]) + t);
+            }
+            g[a] = __builtin_fmaf(om1, g[a], ia);
+            g[b] = __builtin_fmaf(om1, g[b], ib);
+        }
     });
 }
 
