@@ -459,8 +459,7 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
         // (stored there by this node at the previous step, below)
         gather_wave<T, FIRST>(sp, zbc, zoff, zoff_up, zoff_dn, bot_lane, top_lane, ww, v);
 
-        double mass_node, jx_raw, g11, g12, g13, g14;
-        double gx[Q];  // post-collision values of the e_x != 0 directions (budget)
+        double mass_node, jx_raw;
         if constexpr (std::is_same<T, double>::value) {
             double ux, uy, uz, dens;
             moments_fp64(v, ux, uy, uz, dens);
@@ -478,8 +477,6 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
                     acc.sq += (du * du + dv * dv) + dw * dw;
                 }
             }
-#pragma unroll
-            for (int a = 0; a < Q; a++) gx[a] = v[a];
         } else {
             float drho, u0, u1, u2, jx;
             collide_stored(v, (float)A.omega, (float)G, drho, u0, u1, u2, jx);
@@ -496,20 +493,29 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
                     acc.sq += (du * du + dv * dv) + dw * dw;
                 }
             }
-#pragma unroll
-            for (int a = 0; a < Q; a++) gx[a] = WD[a] + (double)Store<T>::to_g(v[a]);
         }
-        g11 = gx[11];
-        g12 = gx[12];
-        g13 = gx[13];
-        g14 = gx[14];
+        // post-collision f of direction a in FP64 (budget terms), formed
+        // only where one is needed
+        auto fpost = [&](auto ac) -> double {
+            constexpr int a = decltype(ac)::value;
+            if constexpr (std::is_same<T, double>::value)
+                return v[a];
+            else
+                return WD[a] + (double)Store<T>::to_g(v[a]);
+        };
+        using I11 = std::integral_constant<int, 11>;
+        using I12 = std::integral_constant<int, 12>;
+        using I13 = std::integral_constant<int, 13>;
+        using I14 = std::integral_constant<int, 14>;
         double mom_node = jx_raw + G;
         // bounce-back reverses e_x of the populations pushed into a
         // no-slip wall node from a no-slip wall column (xcomputations.c:370)
         if (bot_lane)
-            mom_node = mom_node - (((ww >> 10) & 1u) ? 2.0 * g12 : 0.0) + (((ww >> 11) & 1u) ? 2.0 * g13 : 0.0);
+            mom_node = mom_node - (((ww >> 10) & 1u) ? 2.0 * fpost(I12{}) : 0.0) +
+                       (((ww >> 11) & 1u) ? 2.0 * fpost(I13{}) : 0.0);
         if (top_lane)
-            mom_node = mom_node - (((ww >> 12) & 1u) ? 2.0 * g11 : 0.0) + (((ww >> 13) & 1u) ? 2.0 * g14 : 0.0);
+            mom_node = mom_node - (((ww >> 12) & 1u) ? 2.0 * fpost(I11{}) : 0.0) +
+                       (((ww >> 13) & 1u) ? 2.0 * fpost(I14{}) : 0.0);
         if (BANDS && slow && !solid_own) {
             // and so does every push of an e_x != 0 population into a band
             static_for<1, Q>([&](auto ac) {
@@ -517,7 +523,7 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
                 if constexpr (EX[a] != 0) {
                     const int zt = z + EZ[a];
                     if (zt >= 0 && zt < Z && band_solid(bw[1 - EX[a]][1 - EY[a]], zt, Z))
-                        mom_node -= 2.0 * (double)EX[a] * gx[a];
+                        mom_node -= 2.0 * (double)EX[a] * fpost(ac);
                 }
             });
         }
