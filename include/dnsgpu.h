@@ -281,7 +281,8 @@ typedef struct dnsens_member_info {
     int pad;
 } dnsens_member_info;
 
-/* State of member h after the last dnsens_step. */
+/* State of member h: after the last dnsens_step, or as admitted when no
+ * dnsens_step has run since its dnsens_add. */
 int dnsens_info(dnsens *e, const dnsgpu *h, dnsens_member_info *out);
 
 /* The last n log rows of member h from the last chunk of the last
