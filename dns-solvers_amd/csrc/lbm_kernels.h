@@ -218,6 +218,15 @@ __device__ __forceinline__ bool band_solid(uint32_t bw, int zs, int Z)
     return (zs >= 0 && zs < hb) || (zs < Z && zs >= Z - ht);
 }
 
+// Row zs (-1 <= zs <= Z) of a column with band word bw is not fluid: a band
+// row, or one of the wall rows -1 and Z (zs < hb holds for zs = -1 and
+// zs >= Z - ht for zs = Z, whatever the band heights).
+__device__ __forceinline__ bool solid_or_wall(uint32_t bw, int zs, int Z)
+{
+    const int hb = (int)(bw & 0xFFFFu), ht = (int)(bw >> 16);
+    return zs < hb || zs >= Z - ht;
+}
+
 __device__ __forceinline__ uint32_t band_word_lane(const uint32_t* bands, int x, int y, int Y)
 {
     return bands ? bands[x * Y + y] : 0u;
@@ -543,9 +552,10 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
             static_for<0, Q>([&](auto ac) {
                 constexpr int a = decltype(ac)::value;
                 const int zt = z + EZ[a];
-                const bool inside = zt >= 0 && zt < Z;
+                // row zt exists (only the e_z != 0 directions can leave the column)
+                const bool inside = EZ[a] == 0 ? true : (EZ[a] > 0 ? zt < Z : zt >= 0);
                 // x + e_a is not a fluid node (a band node or a wall row)
-                const bool nb_solid = a == 0 ? solid_own : (!inside || band_solid(bw[1 - EX[a]][1 - EY[a]], zt, Z));
+                const bool nb_solid = a == 0 ? solid_own : solid_or_wall(bw[1 - EX[a]][1 - EY[a]], zt, Z);
                 if (!solid_own || nb_solid) *(GLOBAL_AS T*)(dp[1][1] + a * zbc + zoff) = v[a];
                 if (a != 0 && !solid_own && nb_solid && inside)
                     *(GLOBAL_AS T*)(dp[1 - EX[a]][1 - EY[a]] + (OPP[a] * zbc + EZ[a] * (int)sizeof(T)) + zoff) = v[a];
