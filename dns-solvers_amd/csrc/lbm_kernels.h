@@ -298,8 +298,8 @@ __device__ __forceinline__ void load_local19(const T* __restrict__ src, const Co
 }
 
 // Hot-path gather for one wave: lane z of the chunk.  Every source address
-// is a wave-uniform base (SGPRs: upstream column block + direction segment
-// - e_z) plus the lane's row offset.  The wall-adjacent lanes, whose
+// is a wave-uniform base (SGPRs: upstream column block + direction segment)
+// plus the lane's upstream row offset (row z - e_z).  The wall-adjacent lanes, whose
 // upstream row is in the wall, read a clamped in-segment row (so no extra
 // cache line is fetched) and are then overwritten by the wall rule: uniform
 // decisions from the column's wall word, one active lane.
