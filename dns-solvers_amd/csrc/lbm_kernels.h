@@ -227,11 +227,6 @@ __device__ __forceinline__ bool solid_or_wall(uint32_t bw, int zs, int Z)
     return zs < hb || zs >= Z - ht;
 }
 
-__device__ __forceinline__ uint32_t band_word_lane(const uint32_t* bands, int x, int y, int Y)
-{
-    return bands ? bands[x * Y + y] : 0u;
-}
-
 // Gather the post-stream populations of node (column c, row z) from the
 // post-collision array src (pull), wall rule included.  Generic per-thread
 // form used by the cadence kernels.
