@@ -133,6 +133,86 @@ def cpu_baseline(wname, budget_s=12.0, w=None):
             "sample": f"oracle/lbm_oracle.c, {X}x{Y}x{Z} slab of the workload, {n} steps, FP64, 1 thread"}
 
 
+# The reference solver itself as the CPU baseline: oracle/build_ref.sh builds
+# it from its own sources (where they lie, in the build container) for a
+# 4-plane slab of a workload, twice, with ITER = 10 and 110 (the step count
+# is a compile-time macro, definitions.h:52); the binaries travel with the
+# tree.  The per-step cost is the difference of the two runs' wall times
+# over the 100 extra steps, so start-up, the restart read and the step-0
+# outputs cancel.
+REF_ITERS = (10, 110)
+REF_WORKLOADS = ("retau180_768x384x192",)
+
+
+def _ref_case(wname):
+    w = WORKLOADS[wname]
+    omega, F, ub, nu = case_params(w)
+    # the reference's omega = 1/((3/Rebs) ub zdv + 1/2) (dns-const-q-xycut.c:216-217) equals 1/(3 nu + 1/2)
+    return dict(X=4, Y=w["Y"], Z=w["Z"], Rebs=ub * w["Z"] / nu, ub=ub, CFL=0.1)
+
+
+def _ref_exe(wname, iters):
+    return os.path.join(HERE, "oracle", "_ref", f"bench_{wname}_n{iters}", "run-dns")
+
+
+def build_reference_baseline():
+    """Build the reference binaries of the CPU baseline (only where the
+    reference sources are present; a build step, not a measurement)."""
+    import subprocess
+    for wname in REF_WORKLOADS:
+        c = _ref_case(wname)
+        for iters in REF_ITERS:
+            subprocess.run([os.path.join(HERE, "oracle", "build_ref.sh"), f"bench_{wname}_n{iters}", str(c["X"]),
+                            str(c["Y"]), str(c["Z"]), repr(c["Rebs"]), repr(c["ub"]), repr(c["CFL"]), "1", "2",
+                            str(iters)], check=True, capture_output=True)
+
+
+def reference_baseline(wname):
+    """Per-step rate of the reference solver (one MPI rank on one core, its
+    own constant-flow-rate forcing) on the 4-plane slab, from a rest state;
+    None when its binaries or mpirun are absent."""
+    import shutil
+    import subprocess
+    import tempfile
+
+    import numpy as np
+    mpirun = os.environ.get("MPIRUN", "/opt/conda/bin/mpirun")
+    exes = [_ref_exe(wname, n) for n in REF_ITERS] if wname in REF_WORKLOADS else []
+    if not exes or not os.path.exists(mpirun) or not all(os.path.exists(e) for e in exes):
+        return None
+    c = _ref_case(wname)
+    X, Y, Z = c["X"], c["Y"], c["Z"]
+    ft = np.zeros((X, Y, Z + 2, 19))
+    ft[:, :, 1:Z + 1, :] = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+    secs = []
+    tmp = tempfile.mkdtemp(prefix="dnsref_")
+    try:
+        for exe, iters in zip(exes, REF_ITERS):
+            wd = os.path.join(tmp, str(iters))
+            os.makedirs(wd)
+            # restart pair at TSTR = 1 (dns-const-q-xycut.c:405-435): ftemp || f, then the force
+            with open(os.path.join(wd, "lbf.0001"), "wb") as fh:
+                fh.write(ft.tobytes())
+                fh.write(ft.tobytes())
+            with open(os.path.join(wd, "force.0001"), "wb") as fh:
+                fh.write(np.float64(0.0).tobytes())
+            t0 = time.perf_counter()
+            p = subprocess.run([mpirun, "-np", "1", exe, "-p"], cwd=wd, capture_output=True, timeout=600)
+            secs.append(time.perf_counter() - t0)
+            if p.returncode != 0:
+                print(f"# reference baseline run failed (rc {p.returncode})", file=sys.stderr, flush=True)
+                return None
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    per_step = (secs[1] - secs[0]) / (REF_ITERS[1] - REF_ITERS[0])
+    if per_step <= 0:
+        return None
+    return {"value": X * Y * Z / per_step / 1e6, "unit": "MLUPS", "cores": 1, "kind": "reference",
+            "sample": f"the reference solver (DNS-solver/src, built from its sources, 1 MPI rank) on a {X}x{Y}x{Z} "
+                      f"slab of the workload, constant flow rate, from rest: ({REF_ITERS[1]} - {REF_ITERS[0]}) steps "
+                      f"between two runs of {secs[0]:.2f} s and {secs[1]:.2f} s"}
+
+
 def measured_traffic(wname):
     """HBM bytes per launch of the step kernel from the committed rocprofv3
     FETCH_SIZE / WRITE_SIZE summary of this workload (calibrated against a
@@ -352,6 +432,12 @@ def main():
         print("bench.py: this build runs the whole channel on one GPU; multi-GPU runs are not part of it "
               "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
+    # the CPU baselines run before this process touches the GPU (the
+    # reference runs as a child process)
+    cpu = cpu_ref = None
+    if not args.no_cpu and args.workload in WORKLOADS:
+        cpu_ref = reference_baseline(args.workload)
+        cpu = cpu_baseline(args.workload)
     dnsgpu.load()
     if args.workload == SWEEP:
         print(json.dumps(run_sweep(args)))
@@ -367,7 +453,6 @@ def main():
         print(f"# fp32 done: {r32['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
         r16 = run_precision(args.workload, dnsgpu.FP16, args.steps, args.warmup)
         print(f"# fp16 done: {r16['mlups']:.0f} MLUPS", file=sys.stderr, flush=True)
-    cpu = None if args.no_cpu else cpu_baseline(args.workload)
     w = WORKLOADS[args.workload]
     tr = measured_traffic(args.workload)
     t64 = tr["fp64_stats"]["traffic_bytes_per_launch"] if tr else None
@@ -410,7 +495,13 @@ def main():
                                              "peak": PEAK_HBM_GBS, "unit": "GB/s",
                                              "frac": round(r16["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": t16,
                                              "kernel": "step_kernel<_Float16,false,true>"})
-    if cpu:
+    # the reference itself when its binaries are here, else the oracle (a
+    # port of the same loop, bitwise equal to it); both are reported
+    if cpu_ref:
+        out["cpu_baseline"] = cpu_ref
+        if cpu:
+            out["cpu_baseline_port"] = cpu
+    elif cpu:
         out["cpu_baseline"] = cpu
     print(json.dumps(out))
 
