@@ -343,11 +343,12 @@ __device__ __forceinline__ void gather_wave(const gbyte* const (&sp)[3][3], int 
     }
 }
 
-// Dynamic LDS of the step kernel: [W][kNG][kWave] global partials, then
-// with STATS [W][kNQ][Z] per-wave plane sums and [3][Z] shifts.
+// Dynamic LDS of the step kernel: [W][kNG] per-wave global sums, then with
+// STATS [W][kNQ][Z] per-wave plane sums and [3][Z] shifts.  (The LDS bounds
+// the blocks resident per CU: at Z = 192, 4 waves, 29 KB, five blocks.)
 __host__ __device__ inline size_t step_lds_bytes(int Z, int W, bool stats)
 {
-    return sizeof(double) * ((size_t)W * kNG * kWave + (stats ? (size_t)W * kNQ * Z + 3 * (size_t)Z : 0));
+    return sizeof(double) * ((size_t)W * kNG + (stats ? (size_t)W * kNQ * Z + 3 * (size_t)Z : 0));
 }
 
 // Waves per block of the step kernel for a given Z: at most 4, and the
@@ -370,8 +371,8 @@ __device__ __forceinline__ void collide_stored(float (&v)[Q], float omega, float
 __device__ __forceinline__ void collide_stored(_Float16 (&v)[Q], float omega, float G, float& drho, float& u0,
                                                float& u1, float& u2, float& jx)
 {
-    // collide the stored values g 2^16 directly (bitwise the result of
-    // unscaling, colliding and scaling back)
+    // collide the stored values g 2^16 directly (the collision's weights
+    // carry the 2^16)
     float g[Q];
 #pragma unroll
     for (int a = 0; a < Q; a++) g[a] = (float)v[a];
@@ -576,11 +577,18 @@ __device__ __forceinline__ void step_block_reduce(double* red, const double* pla
                                                   int W, int wave, int lane, double* part, double* gpart, int nslot,
                                                   int slot)
 {
-    double* myred = red + (size_t)wave * kNG * kWave;
-    myred[0 * kWave + lane] = acc.jx;
-    myred[1 * kWave + lane] = acc.mass;
-    myred[2 * kWave + lane] = acc.mom;
-    myred[3 * kWave + lane] = acc.sq;
+    // each wave's global sums over its lanes (fixed butterfly), then over the
+    // block's waves in wave order
+    double g[kNG] = {acc.jx, acc.mass, acc.mom, acc.sq};
+#pragma unroll
+    for (int q = 0; q < kNG; q++) {
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) g[q] += __shfl_xor(g[q], off, 64);
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int q = 0; q < kNG; q++) red[wave * kNG + q] = g[q];
+    }
     __syncthreads();
     if constexpr (STATS) {
         for (int i = threadIdx.x; i < kNQ * Z; i += blockDim.x) {
@@ -590,15 +598,12 @@ __device__ __forceinline__ void step_block_reduce(double* red, const double* pla
             part[((long long)q * Z + z) * nslot + slot] = s;
         }
     }
-    if (wave == 0) {
-#pragma unroll
-        for (int q = 0; q < kNG; q++) {
-            if (!STATS && (q == 0 || q == 3)) continue;
+    if (threadIdx.x < kNG) {
+        const int q = threadIdx.x;
+        if (STATS || (q != 0 && q != 3)) {
             double s = 0.0;
-            for (int w = 0; w < W; w++) s += red[((size_t)w * kNG + q) * kWave + lane];
-#pragma unroll
-            for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
-            if (lane == 0) gpart[(long long)q * nslot + slot] = s;
+            for (int w = 0; w < W; w++) s += red[w * kNG + q];
+            gpart[(long long)q * nslot + slot] = s;
         }
     }
 }
@@ -613,8 +618,8 @@ static __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs 
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int W = blockDim.x / kWave;
     const int Z = A.g.Z;
-    double* red = lds;  // [W][kNG][kWave]
-    double* planes = lds + (size_t)W * kNG * kWave;
+    double* red = lds;  // [W][kNG]
+    double* planes = lds + (size_t)W * kNG;
     double* mine = planes + (size_t)wave * kNQ * Z;
     double* shl = planes + (size_t)W * kNQ * Z;
 
