@@ -147,6 +147,40 @@ def test_restart_equals_uninterrupted():
     assert np.array_equal(end1, end2)
 
 
+@pytest.mark.parametrize("precision", [dnsgpu.FP32, dnsgpu.FP16])
+def test_restart_equals_uninterrupted_reduced_storage(precision):
+    """A checkpoint of an FP32- or FP16-shifted state (written as FP64
+    f = w + g, the reference's lbf layout) reloads to the same stored values:
+    the restarted run continues bitwise as the uninterrupted one (constant F,
+    riblets included).  The stored values convert exactly to FP64 and back:
+    w + g is exact in FP64 for an FP16 g·2^16 (11 significant bits) and for
+    an FP32 g that is not smaller than about 2^-29 w, as everywhere in this
+    flow."""
+    X, Y, Z = 16, 16, 24
+    print(f"case: restart, precision {precision}", flush=True)
+    hb = fields.riblet_bands(X, Y, 8, 4)
+
+    def make():
+        ch = dnsgpu.Channel(X, Y, Z, Rebs=60.0, ub=0.05, CFL=0.25, TSTR=100, forcing=dnsgpu.CONST_F, force=1e-5,
+                            precision=precision)
+        ch.set_solid_bands(hb, hb)
+        return ch
+
+    ch = make()
+    ch.init_field(dnsgpu.INIT_POISEUILLE, uc=0.06, amp=0.05, seed=3)
+    ch.step(20, logs=False)
+    a, b = ch.save_aos()
+    ch.step(30, logs=False)
+    end1 = ch.save_aos()[0]
+    ch2 = make()
+    ch2.load_aos(a, b)
+    ch2.step(30, logs=False)
+    end2 = ch2.save_aos()[0]
+    assert np.array_equal(end1, end2), rel(end2, end1)
+    ch.close()
+    ch2.close()
+
+
 def test_fp32_shifted_against_fp64():
     X, Y, Z = 16, 8, 24
     print("case: fp32 vs fp64", flush=True)
