@@ -115,6 +115,34 @@ def test_driver_riblets(tmp_path):
     assert (den[:, 4, 1:4] == 0).all() and abs(den[:, 0, 1] - 1).max() < 0.1
 
 
+@pytest.mark.parametrize("precision,tol", [("fp32", 1e-5), ("fp16", 1e-3)])
+def test_driver_reduced_storage_runs(tmp_path, precision, tol):
+    """The driver with `precision = fp32 | fp16` writes the same file set as
+    the FP64 run of the case, with logs and the final checkpoint within the
+    storage's bar of the FP64 ones (constant force, riblets)."""
+    p = dict(Rebs=60.0, ub=0.05, X=8, Y=32, Z=24, TSTR=0, CFL=0.25, N=48, slip=False)
+    extra = ("forcing = constf\nforce = 1e-5\ntexture = riblets\nriblet_spacing = 8\nriblet_height = 4\n"
+             "init = poiseuille\ninit_uc = 0.06\ninit_amp = 0.05\n")
+    runs = {}
+    for prec in ("fp64", precision):
+        d = tmp_path / prec
+        d.mkdir()
+        write_case(d / "dns.case", p, extra + f"precision = {prec}\n")
+        r = run(["-s"], d)
+        assert r.returncode == 0, r.stderr
+        runs[prec] = d
+    a, b = runs["fp64"], runs[precision]
+    assert sorted(x.name for x in a.iterdir() if x.name != "dns.case") == \
+        sorted(x.name for x in b.iterdir() if x.name != "dns.case")
+    for log in ("w-str.txt", "flowrate.txt", "energy.txt"):
+        la, lb = np.loadtxt(a / log), np.loadtxt(b / log)
+        assert np.array_equal(la[:, 0], lb[:, 0]), log
+        assert np.abs(lb[:, 1] - la[:, 1]).max() <= tol * np.abs(la[:, 1]).max(), log
+    last = sorted(x.name for x in a.iterdir() if x.name.startswith("lbf."))[-1]
+    fa, fb = np.fromfile(a / last), np.fromfile(b / last)
+    assert np.abs(fb - fa).max() <= tol * np.abs(fa).max(), last
+
+
 def test_driver_rejects_truncated_checkpoint(tmp_path):
     p = dict(Rebs=60.0, ub=0.05, X=8, Y=8, Z=8, TSTR=3, CFL=0.25, N=16, slip=False)
     write_case(tmp_path / "dns.case", p)
