@@ -521,22 +521,6 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
         if (top_lane)
             mom_node = mom_node - (((ww >> 12) & 1u) ? 2.0 * fpost(I11{}) : 0.0) +
                        (((ww >> 13) & 1u) ? 2.0 * fpost(I14{}) : 0.0);
-        if (BANDS && slow && !solid_own) {
-            // and so does every push of an e_x != 0 population into a band
-            static_for<1, Q>([&](auto ac) {
-                constexpr int a = decltype(ac)::value;
-                if constexpr (EX[a] != 0) {
-                    const int zt = z + EZ[a];
-                    if (zt >= 0 && zt < Z && band_solid(bw[1 - EX[a]][1 - EY[a]], zt, Z))
-                        mom_node -= 2.0 * (double)EX[a] * fpost(ac);
-                }
-            });
-        }
-        if (!solid_own) {
-            if constexpr (STATS) acc.jx += jx_raw;
-            acc.mass += mass_node;
-            acc.mom += mom_node;
-        }
         if (BANDS && slow) {
             // Halfway bounce-back at band surfaces, on the write side: the
             // population a fluid node x pushes toward a solid node t = x + e_a
@@ -544,7 +528,8 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
             // slot of direction opp(a), the slot x pulls from.  Every other
             // slot of t is stored by t itself (values no fluid node reads):
             // each slot is written once per step and every cache line of
-            // the destination is written whole.
+            // the destination is written whole.  Each such push also reverses
+            // e_x of that population in the budget (direction order).
             static_for<0, Q>([&](auto ac) {
                 constexpr int a = decltype(ac)::value;
                 const int zt = z + EZ[a];
@@ -553,8 +538,10 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
                 // x + e_a is not a fluid node (a band node or a wall row)
                 const bool nb_solid = a == 0 ? solid_own : solid_or_wall(bw[1 - EX[a]][1 - EY[a]], zt, Z);
                 if (!solid_own || nb_solid) *(GLOBAL_AS T*)(dp[1][1] + a * zbc + zoff) = v[a];
-                if (a != 0 && !solid_own && nb_solid && inside)
+                if (a != 0 && !solid_own && nb_solid && inside) {
                     *(GLOBAL_AS T*)(dp[1 - EX[a]][1 - EY[a]] + (OPP[a] * zbc + EZ[a] * (int)sizeof(T)) + zoff) = v[a];
+                    if constexpr (EX[a] != 0) mom_node -= 2.0 * (double)EX[a] * fpost(ac);
+                }
             });
         } else {
             // the row offset as a 32-bit value of this block, so the stores
@@ -565,6 +552,11 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
                 constexpr int a = decltype(ac)::value;
                 *(GLOBAL_AS T*)(dp[1][1] + a * zbc + zo) = v[a];
             });
+        }
+        if (!solid_own) {
+            if constexpr (STATS) acc.jx += jx_raw;
+            acc.mass += mass_node;
+            acc.mom += mom_node;
         }
     }
 }
