@@ -562,8 +562,9 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
 }
 
 // Deterministic reductions of one block: plane sums over the block's waves
-// in wave order, global sums over waves then a fixed butterfly over lanes;
-// written as partial number `slot` of [kNQ][Z][nslot] and [kNG][nslot].
+// in wave order, global sums by a fixed butterfly over each wave's lanes and
+// then over the waves in order; written as partial number `slot` of
+// [kNQ][Z][nslot] and [kNG][nslot].
 template <bool STATS>
 __device__ __forceinline__ void step_block_reduce(double* red, const double* planes, const StepAcc& acc, int Z,
                                                   int W, int wave, int lane, double* part, double* gpart, int nslot,
