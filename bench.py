@@ -367,6 +367,8 @@ def run_sweep(args):
     nodes = sum(r.xdv * r.ydv * r.zdv for r in reqs)
     bpn = 152 if prec == dnsgpu.FP32 else 76
     achieved = nodes * bpn / (tk / n / 1e3) / 1e9
+    tr = measured_traffic(SWEEP)
+    traffic = tr[PNAME[prec]]["traffic_bytes_per_launch"] if tr and PNAME[prec] in tr else None
     status = {str(k): sum(1 for r in res if r["status"] == k) for k in (1, 0, -1, -2)}
     out = {
         "metric": "MLUPS + % HBM roofline (FP32 vs FP64 D3Q19); ensemble solves/h + p99 latency",
@@ -401,7 +403,8 @@ def run_sweep(args):
         "speedup_vs_sequential_fp64": None if seq is None else round(seq / summ["wall"], 3),
         "sequential_fp64_queue": fifo,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": None,
+                     "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+                     "traffic_source": f"profiles/traffic_{SWEEP}.json" if traffic else None,
                      "kernel": "ens_step_kernel<%s>" % ("float" if prec == dnsgpu.FP32 else "_Float16"),
                      "kernel_ms": round(tk / n, 4), "bytes_per_node": bpn,
                      "nodes": nodes, "grid_blocks": blocks},
