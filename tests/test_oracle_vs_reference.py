@@ -1,8 +1,9 @@
 """The oracle against the reference solver compiled from its own sources in
 this container (oracle/build_ref.sh): bitwise checkpoint state and
 byte-identical log files on grids and seeds the golden fixtures do not
-cover.  Skipped where the reference sources or MPICH are absent (the GPU
-box); tests/test_oracle_golden.py covers that case from fixtures."""
+cover.  Skipped where the reference sources or MPICH are absent (a
+machine that only receives this repository); tests/test_oracle_golden.py
+covers that case from fixtures."""
 import os
 
 import numpy as np
