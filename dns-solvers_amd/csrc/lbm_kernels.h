@@ -343,12 +343,20 @@ __device__ __forceinline__ void gather_wave(const gbyte* const (&sp)[3][3], int 
     }
 }
 
+// The statistics variant keeps the [3][Z] plane-mean shifts in LDS up to
+// Z = 256 and reads them from global memory above: the LDS bounds the blocks
+// resident per CU, and at Z = 384 one more block per CU is worth more than
+// the faster reads (+2% FP32, +3-8% FP16 at Z = 384; at Z = 192 the LDS
+// copy is 3-5% faster).
+__host__ __device__ inline bool step_shifts_in_lds(int Z) { return Z <= 256; }
+
 // Dynamic LDS of the step kernel: [W][kNG] per-wave global sums, then with
-// STATS [W][kNQ][Z] per-wave plane sums and [3][Z] shifts.  (The LDS bounds
-// the blocks resident per CU: at Z = 192, 4 waves, 29 KB, five blocks.)
+// STATS [W][kNQ][Z] per-wave plane sums and (see above) the [3][Z] shifts.
+// At Z = 192, 4 waves: 29 KB, five blocks per CU.
 __host__ __device__ inline size_t step_lds_bytes(int Z, int W, bool stats)
 {
-    return sizeof(double) * ((size_t)W * kNG + (stats ? (size_t)W * kNQ * Z + 3 * (size_t)Z : 0));
+    return sizeof(double) *
+           ((size_t)W * kNG + (stats ? (size_t)W * kNQ * Z + (step_shifts_in_lds(Z) ? 3 * (size_t)Z : 0) : 0));
 }
 
 // Waves per block of the step kernel for a given Z: at most 4, and the
@@ -615,16 +623,24 @@ static __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs 
     double* planes = lds + (size_t)W * kNG;
     double* mine = planes + (size_t)wave * kNQ * Z;
     double* shl = planes + (size_t)W * kNQ * Z;
+    const bool shl_lds = step_shifts_in_lds(Z);
 
     if constexpr (STATS) {
         for (int i = lane; i < kNQ * Z; i += kWave) mine[i] = 0.0;
-        for (int i = threadIdx.x; i < 3 * Z; i += blockDim.x) shl[i] = A.shift[i];
+        if (shl_lds)
+            for (int i = threadIdx.x; i < 3 * Z; i += blockDim.x) shl[i] = A.shift[i];
         __syncthreads();
     }
     StepAcc acc;
     const double G = *A.G;
-    for (int it = blockIdx.x * W + wave; it < A.g.ncols; it += gridDim.x * W)
-        step_column<T, FIRST, STATS, BANDS>(A, __builtin_amdgcn_readfirstlane(it), lane, G, shl, mine, acc);
+    if (!STATS || shl_lds) {
+        for (int it = blockIdx.x * W + wave; it < A.g.ncols; it += gridDim.x * W)
+            step_column<T, FIRST, STATS, BANDS>(A, __builtin_amdgcn_readfirstlane(it), lane, G, shl, mine, acc);
+    } else {
+        // the same loop with the shifts read from global memory
+        for (int it = blockIdx.x * W + wave; it < A.g.ncols; it += gridDim.x * W)
+            step_column<T, FIRST, STATS, BANDS>(A, __builtin_amdgcn_readfirstlane(it), lane, G, A.shift, mine, acc);
+    }
     step_block_reduce<STATS>(red, planes, acc, Z, W, wave, lane, A.part, A.gpart, A.nblk, blockIdx.x);
 }
 
