@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "handle.h"
@@ -33,7 +34,21 @@ struct Member {
 struct TuneEntry {
     int prec, zmax, tiles_log2, blocks_per_cu;
 };
+// shared by every ensemble of the process (ensembles may live on different
+// host threads): guarded
 std::vector<TuneEntry> g_ens_tune;
+std::mutex g_ens_tune_mutex;
+
+bool ens_tune_lookup(int prec, int zmax, int tiles_log2, int* bpc)
+{
+    std::lock_guard<std::mutex> lock(g_ens_tune_mutex);
+    for (auto& t : g_ens_tune)
+        if (t.prec == prec && t.zmax == zmax && t.tiles_log2 == tiles_log2) {
+            if (bpc) *bpc = t.blocks_per_cu;
+            return true;
+        }
+    return false;
+}
 
 int ilog2(int v)
 {
@@ -362,9 +377,7 @@ static int upload_tables(dnsens* e)
     }
     if (!getenv("DNSENS_BLOCKS_PER_CU")) {
         e->blocks_per_cu = default_blocks_per_cu(e->precision);
-        for (auto& t : g_ens_tune)
-            if (t.prec == e->precision && t.zmax == e->zmax && t.tiles_log2 == ilog2(std::max(1, e->ntiles)))
-                e->blocks_per_cu = t.blocks_per_cu;
+        ens_tune_lookup(e->precision, e->zmax, ilog2(std::max(1, e->ntiles)), &e->blocks_per_cu);
     }
     const int old_grid = e->grid;
     pick_grid(e);
@@ -557,6 +570,7 @@ static int tune_now(dnsens* e, int trials, int* best_blocks, double* best_ms)
     if (!e->mem.empty()) {
         int rc = with_storage(e->precision, [&](auto t) { return tune_grid<decltype(t)>(e, trials, &bpc, &ms); });
         if (rc) return rc;
+        std::lock_guard<std::mutex> lock(g_ens_tune_mutex);
         g_ens_tune.push_back({e->precision, e->zmax, ilog2(std::max(1, e->ntiles)), bpc});
     }
     e->layout_changed = true;
@@ -568,8 +582,7 @@ static int tune_now(dnsens* e, int trials, int* best_blocks, double* best_ms)
 static int maybe_autotune(dnsens* e)
 {
     if (!(e->flags & DNSENS_AUTOTUNE) || getenv("DNSENS_BLOCKS_PER_CU")) return 0;
-    for (auto& t : g_ens_tune)
-        if (t.prec == e->precision && t.zmax == e->zmax && t.tiles_log2 == ilog2(std::max(1, e->ntiles))) return 0;
+    if (ens_tune_lookup(e->precision, e->zmax, ilog2(std::max(1, e->ntiles)), nullptr)) return 0;
     return tune_now(e, 3, nullptr, nullptr);
 }
 

@@ -14,6 +14,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "handle.h"
@@ -393,10 +394,14 @@ struct TuneKey {
         return X == o.X && Y == o.Y && Z == o.Z && prec == o.prec && bands == o.bands;
     }
 };
+// shared by every handle of the process (handles may live on different host
+// threads): guarded
 std::vector<std::pair<TuneKey, int>> g_tune_cache;
+std::mutex g_tune_mutex;
 
 bool tune_lookup(const TuneKey& k, int* blocks)
 {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
     for (auto& e : g_tune_cache)
         if (e.first == k) {
             *blocks = e.second;
@@ -421,6 +426,7 @@ bool tune_lookup(const TuneKey& k, int* blocks)
 
 void tune_store(const TuneKey& k, int blocks)
 {
+    std::lock_guard<std::mutex> lock(g_tune_mutex);
     g_tune_cache.push_back({k, blocks});
     const char* path = getenv("DNSGPU_TUNE_CACHE");
     if (!path) return;
