@@ -503,14 +503,18 @@ extern "C" int dnsgpu_set_walls(dnsgpu* h, const int8_t* bottom, const int8_t* t
     const size_t xy = (size_t)h->g.X * h->g.Y;
     const int8_t* src[2] = {bottom, top};
     int8_t* dst[2] = {h->mbot, h->mtop};
+    // both walls are checked before either is copied: a refused call changes nothing
+    for (int w = 0; w < 2; w++) {
+        if (!src[w]) continue;
+        for (size_t i = 0; i < xy; i++) {
+            if (src[w][i] != -1 && src[w][i] != -2)
+                return set_err(DNSGPU_EARG, "wall code %d at %zu (must be -1 or -2)", src[w][i], i);
+            if (h->has_bands && src[w][i] == -2)
+                return set_err(DNSGPU_EARG, "slip wall nodes cannot be combined with solid bands");
+        }
+    }
     for (int w = 0; w < 2; w++) {
         if (src[w]) {
-            for (size_t i = 0; i < xy; i++) {
-                if (src[w][i] != -1 && src[w][i] != -2)
-                    return set_err(DNSGPU_EARG, "wall code %d at %zu (must be -1 or -2)", src[w][i], i);
-                if (h->has_bands && src[w][i] == -2)
-                    return set_err(DNSGPU_EARG, "slip wall nodes cannot be combined with solid bands");
-            }
             HIPCHK(hipMemcpyAsync(dst[w], src[w], xy, hipMemcpyHostToDevice, h->stream));
         } else {
             HIPCHK(hipMemsetAsync(dst[w], -1, xy, h->stream));
