@@ -229,12 +229,23 @@ static int plan_handle(const dns_case* c, dnsgpu* h)
     // omega and the constants exactly as dns-const-q-xycut.c:211-217,
     // wshearstr.c:24, encalc.c:20 evaluate them.
     const int zd = g.Z + 2;
+    if (!std::isfinite(c->ub) || !std::isfinite(c->force))
+        return set_err(DNSGPU_EARG, "ub %g / force %g not finite", c->ub, c->force);
+    if (c->omega < 0) return set_err(DNSGPU_EARG, "omega %g negative (0 derives it from Rebs)", c->omega);
     if (c->omega > 0)
         h->omega = c->omega;
     else {
         double tau = (3. / c->Rebs) * c->ub * (zd - 2.) + 0.5;
         h->omega = 1. / tau;
     }
+    // BGK is stable only for 0 < omega < 2 (tau > 1/2); the reference takes
+    // any input and runs into NaN instead
+    if (!(h->omega > 0.0 && h->omega < 2.0))
+        return set_err(DNSGPU_EARG, "relaxation rate omega = %g outside (0, 2) (Rebs %g, ub %g, omega %g)", h->omega,
+                       c->Rebs, c->ub, c->omega);
+    if (!(c->CFL > 0.0) || !std::isfinite(c->CFL)) return set_err(DNSGPU_EARG, "CFL %g must be positive", c->CFL);
+    if (c->forcing != DNSGPU_CONST_Q && c->forcing != DNSGPU_CONST_F)
+        return set_err(DNSGPU_EARG, "forcing %d unknown", c->forcing);
     h->mu = (1. / 6.) * (2. / h->omega - 1);
     h->dt = 1. / ((double)(0.5 * (zd - 2.) / c->CFL));
 

@@ -108,7 +108,11 @@ typedef struct dnsgpu dnsgpu;
 
 /* Allocate the device state for a case.  Replaces the calloc block of
  * dns-const-q-xycut.c:239-356 (no halo buffers: the periodic x/y wrap is
- * done inside the kernel). */
+ * done inside the kernel).  DNSGPU_EARG, with *out = NULL, for a grid
+ * outside xdv, ydv >= 2, 4 <= zdv <= 512, fewer than 2^31 nodes, or for
+ * parameters the reference would run into NaN with: a relaxation rate
+ * (given, or derived from Rebs and ub) outside (0, 2), CFL <= 0, a
+ * non-finite ub or force, an unknown forcing or precision. */
 int dnsgpu_create(const dns_case *c, dnsgpu **out);
 void dnsgpu_destroy(dnsgpu *h);
 const char *dnsgpu_last_error(void);

@@ -41,6 +41,26 @@ def test_create_refuses_unsupported_grids(grid, why):
     assert L.dnsens_footprint(ctypes.byref(c)) == 0
 
 
+@pytest.mark.parametrize("field,value,why", [
+    ("Rebs", -50.0, "omega"),          # tau < 1/2
+    ("Rebs", 0.0, "omega"),            # tau infinite
+    ("omega", 2.0, "omega"),           # the BGK stability limit
+    ("omega", -1.0, "negative"),
+    ("CFL", 0.0, "CFL"),
+    ("forcing", 2, "forcing 2"),
+    ("ub", float("nan"), "not finite"),
+    ("force", float("inf"), "not finite"),
+])
+def test_create_refuses_unphysical_parameters(field, value, why):
+    L = lib()
+    c = case()
+    setattr(c, field, value)
+    h = ctypes.c_void_p()
+    assert L.dnsgpu_create(ctypes.byref(c), ctypes.byref(h)) == EARG
+    assert why in L.dnsgpu_last_error().decode()
+    assert L.dnsgpu_footprint(ctypes.byref(c)) == 0
+
+
 def test_create_refuses_unknown_precision():
     L = lib()
     c = case(precision=7)
