@@ -460,8 +460,18 @@ extern "C" int dnsgpu_autotune(dnsgpu* h, int trial_steps, int* best_blocks, dou
         if (best_ms) *best_ms = -1.0;
         return 0;
     }
-    const int saved_ps = h->poststream;
-    h->poststream = 0;  // time the steady-state kernel
+    // time the steady-state kernel; the handle's state flag and grid are
+    // restored on every return
+    struct Restore {
+        dnsgpu* h;
+        int ps, nblk;
+        ~Restore()
+        {
+            h->poststream = ps;
+            h->step_nblk = nblk;
+        }
+    } restore{h, h->poststream, h->step_nblk};
+    h->poststream = 0;
     const int wpcs[] = {2, 3, 4, 6, 8, 12, 16, 20, 24, 32};
     int best = h->step_nblk;
     double best_t = 1e30;
@@ -488,8 +498,7 @@ extern "C" int dnsgpu_autotune(dnsgpu* h, int trial_steps, int* best_blocks, dou
             best = nb;
         }
     }
-    h->poststream = saved_ps;
-    h->step_nblk = best;
+    restore.nblk = best;
     tune_store(key, best);
     if (best_blocks) *best_blocks = best;
     if (best_ms) *best_ms = best_t;
