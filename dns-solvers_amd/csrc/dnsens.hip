@@ -203,6 +203,7 @@ extern "C" int dnsens_add(dnsens* e, dnsgpu* h)
         return set_err(DNSGPU_EARG, "member precision %d differs from the ensemble's %d", h->c.precision,
                        e->precision);
     if (find_member(e, h) >= 0) return set_err(DNSGPU_EARG, "handle is already a member");
+    if (h->owner) return set_err(DNSGPU_EARG, "handle is a member of another ensemble");
     if ((int)e->mem.size() >= DNSENS_MAX_MEMBERS)
         return set_err(DNSGPU_EARG, "ensemble holds %d members already", DNSENS_MAX_MEMBERS);
     if (h->g.Z > kEnsZmax)
@@ -222,22 +223,25 @@ extern "C" int dnsens_add(dnsens* e, dnsgpu* h)
         free_member(m);
         return set_err(DNSGPU_ENOMEM, "tile records of %d tiles", m.ntiles);
     }
+    auto fail = [&](int code) {
+        free_member(m);
+        return code;
+    };
     // a state fresh from a load or an init holds post-stream populations:
     // its first step (local gather) runs through the single-case kernels
     DevState st;
-    HIPCHK(hipMemcpyAsync(&st, h->st, sizeof st, hipMemcpyDeviceToHost, h->stream));
-    HIPCHK(hipStreamSynchronize(h->stream));
+    if (hipMemcpyAsync(&st, h->st, sizeof st, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess)
+        return fail(set_err(DNSGPU_EHIP, "reading the member's state failed"));
     double stress = 0.0;
     if (h->poststream) {
         int rc = do_step(h, true, h->ring, (long long)st.steps);
-        if (rc) {
-            free_member(m);
-            return rc;
-        }
+        if (rc) return fail(rc);
         LogRow row;
-        HIPCHK(hipMemcpyAsync(&st, h->st, sizeof st, hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipMemcpyAsync(&row, h->ring, sizeof row, hipMemcpyDeviceToHost, h->stream));
-        HIPCHK(hipStreamSynchronize(h->stream));
+        if (hipMemcpyAsync(&st, h->st, sizeof st, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+            hipMemcpyAsync(&row, h->ring, sizeof row, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+            hipStreamSynchronize(h->stream) != hipSuccess)
+            return fail(set_err(DNSGPU_EHIP, "reading the member's first step failed"));
         stress = row.stress;
     }
     // the member's state as of now (dnsens_info before its first dnsens_step)

@@ -261,8 +261,10 @@ size_t dnsens_footprint(const dns_case *c);
 
 /* Admit handle h (initialised or loaded, same precision).
  * Returns DNSGPU_ENOMEM, with nothing changed, when its bytes do not fit
- * the budget.  A state fresh from an init or a load takes its first step
- * here (one logged step). */
+ * the budget, and DNSGPU_EARG when h is a member already (of this or of
+ * another ensemble: a handle has one owner until dnsens_remove).  A state
+ * fresh from an init or a load takes its first step here (one logged
+ * step). */
 int dnsens_add(dnsens *e, dnsgpu *h);
 /* The handle leaves the ensemble (state as of the last dnsens_step). */
 int dnsens_remove(dnsens *e, dnsgpu *h);

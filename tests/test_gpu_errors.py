@@ -83,3 +83,21 @@ def test_bands_and_slip_walls_exclude_each_other():
         refused(c.set_walls, None, slip, match="cannot be combined")
         c.set_walls(None, None)  # no-slip walls remain allowed
         c.step(2)
+
+
+def test_a_handle_belongs_to_one_ensemble():
+    print("case: one owner per handle", flush=True)
+    ch = seeded(dnsgpu.FP32)
+    other = seeded(dnsgpu.FP32)
+    ea, eb = dnsgpu.Ensemble(precision=dnsgpu.FP32), dnsgpu.Ensemble(precision=dnsgpu.FP32)
+    ea.add(ch)
+    refused(ea.add, ch, match="already a member")
+    refused(eb.add, ch, match="another ensemble")
+    refused(eb.remove, other, match="not a member")
+    assert eb.count() == 0 and eb.bytes_used() == 0
+    ea.step(3)
+    ea.remove(ch)
+    eb.add(ch)  # free again once removed
+    eb.step(2)
+    for x in (ea, eb, ch, other):
+        x.close()
