@@ -417,6 +417,37 @@ def run_sweep(args):
     return out
 
 
+def run_z_sweep(args):
+    """Step-kernel bandwidth against the wall-normal size Z at about the
+    Re_tau 180 node count (Y = 384, X = 56623104 // (384 Z)), smooth walls,
+    each precision autotuned, HIP-event kernel times over --steps logged
+    steps: how the 64-row chunking and the direction-segment length
+    (Z * sizeof(T) bytes) bear on each storage precision."""
+    zs = [int(z) for z in args.z_list.split(",")]
+    precs = [{"fp64": dnsgpu.FP64, "fp32": dnsgpu.FP32, "fp16": dnsgpu.FP16}[p] for p in args.z_prec.split(",")]
+    out = []
+    for Z in zs:
+        Y = 384
+        X = max(2, 56623104 // (Y * Z))
+        for prec in precs:
+            ch = dnsgpu.Channel(X, Y, Z, Rebs=3000.0, ub=0.1, CFL=0.1, forcing=dnsgpu.CONST_Q, precision=prec)
+            try:
+                ch.init_field(dnsgpu.INIT_TURBULENT, uc=0.006, amp=0.1, seed=1, nmodes=2)
+                ch.step(2, logs=False)
+                blocks, _ = ch.autotune(trial_steps=3)
+                ms_k, _ = ch.time_steps(args.steps, stats=True)
+            finally:
+                ch.close()
+            esize = {dnsgpu.FP64: 8, dnsgpu.FP32: 4, dnsgpu.FP16: 2}[prec]
+            k = ms_k / args.steps
+            gbs = 2 * 19 * esize * X * Y * Z / (k / 1e3) / 1e9
+            out.append({"Z": Z, "X": X, "Y": Y, "precision": PNAME[prec], "blocks": blocks,
+                        "kernel_us": round(k * 1e3, 1), "GBs": round(gbs), "frac": round(gbs / PEAK_HBM_GBS, 4)})
+            print(f"# Z={Z} {PNAME[prec]}: {k * 1e3:.0f} us, {gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    return {"metric": "step-kernel GB/s against Z", "unit": "GB/s", "n_gpus": 1, "steps": args.steps,
+            "data": "synthetic", "z_sweep": out}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -430,6 +461,9 @@ def main():
     ap.add_argument("--sweep-fp16", action="store_true", help=f"{SWEEP}: FP16-shifted storage")
     ap.add_argument("--dr-steps", type=int, default=0, help="drag workloads: steps per run")
     ap.add_argument("--dr-avg-from", type=int, default=0, help="drag workloads: first averaged step")
+    ap.add_argument("--z-sweep", action="store_true", help="step-kernel GB/s against Z (see run_z_sweep)")
+    ap.add_argument("--z-list", default="64,96,128,192,384")
+    ap.add_argument("--z-prec", default="fp64,fp32,fp16")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs the whole channel on one GPU; multi-GPU runs are not part of it "
@@ -438,10 +472,13 @@ def main():
     # the CPU baselines run before this process touches the GPU (the
     # reference runs as a child process)
     cpu = cpu_ref = None
-    if not args.no_cpu and args.workload in WORKLOADS:
+    if not args.no_cpu and args.workload in WORKLOADS and not args.z_sweep:
         cpu_ref = reference_baseline(args.workload)
         cpu = cpu_baseline(args.workload)
     dnsgpu.load()
+    if args.z_sweep:
+        print(json.dumps(run_z_sweep(args)))
+        return
     if args.workload == SWEEP:
         print(json.dumps(run_sweep(args)))
         return
