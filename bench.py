@@ -142,17 +142,23 @@ def cpu_baseline(wname, budget_s=12.0, w=None):
 # outputs cancel.
 REF_ITERS = (10, 110)
 REF_WORKLOADS = ("retau180_768x384x192",)
+# the reference's own parallel mode beside it: x-slabs over REF_MPI_RANKS
+# ranks (the host cores a one-GPU share of the machine has), two planes each
+REF_MPI_RANKS = 16
+REF_MPI_PLANES = 2
 
 
-def _ref_case(wname):
+def _ref_case(wname, ranks=1):
     w = WORKLOADS[wname]
     omega, F, ub, nu = case_params(w)
     # the reference's omega = 1/((3/Rebs) ub zdv + 1/2) (dns-const-q-xycut.c:216-217) equals 1/(3 nu + 1/2)
-    return dict(X=4, Y=w["Y"], Z=w["Z"], Rebs=ub * w["Z"] / nu, ub=ub, CFL=0.1)
+    X = 4 if ranks == 1 else REF_MPI_PLANES * ranks
+    return dict(X=X, Y=w["Y"], Z=w["Z"], Rebs=ub * w["Z"] / nu, ub=ub, CFL=0.1)
 
 
-def _ref_exe(wname, iters):
-    return os.path.join(HERE, "oracle", "_ref", f"bench_{wname}_n{iters}", "run-dns")
+def _ref_exe(wname, iters, ranks=1):
+    tag = "" if ranks == 1 else f"_p{ranks}"
+    return os.path.join(HERE, "oracle", "_ref", f"bench_{wname}{tag}_n{iters}", "run-dns")
 
 
 def build_reference_baseline():
@@ -160,27 +166,32 @@ def build_reference_baseline():
     reference sources are present; a build step, not a measurement)."""
     import subprocess
     for wname in REF_WORKLOADS:
-        c = _ref_case(wname)
-        for iters in REF_ITERS:
-            subprocess.run([os.path.join(HERE, "oracle", "build_ref.sh"), f"bench_{wname}_n{iters}", str(c["X"]),
-                            str(c["Y"]), str(c["Z"]), repr(c["Rebs"]), repr(c["ub"]), repr(c["CFL"]), "1", "2",
-                            str(iters)], check=True, capture_output=True)
+        for ranks in (1, REF_MPI_RANKS):
+            c = _ref_case(wname, ranks)
+            tag = "" if ranks == 1 else f"_p{ranks}"
+            for iters in REF_ITERS:
+                subprocess.run([os.path.join(HERE, "oracle", "build_ref.sh"), f"bench_{wname}{tag}_n{iters}",
+                                str(c["X"]), str(c["Y"]), str(c["Z"]), repr(c["Rebs"]), repr(c["ub"]), repr(c["CFL"]),
+                                "1", "2", str(iters)], check=True, capture_output=True,
+                               env=dict(os.environ, REF_XDIM=str(ranks)))
 
 
-def reference_baseline(wname):
-    """Per-step rate of the reference solver (one MPI rank on one core, its
-    own constant-flow-rate forcing) on the 4-plane slab, from a rest state;
-    None when its binaries or mpirun are absent."""
+def reference_baseline(wname, ranks=1):
+    """Per-step rate of the reference solver (its own constant-flow-rate
+    forcing, from a rest state): one MPI rank on one core on the 4-plane
+    slab, or `ranks` ranks splitting a 2-planes-per-rank slab in x (its
+    parallel mode, throughput only: each rank exchanges its x faces with its
+    neighbours every step); None when its binaries or mpirun are absent."""
     import shutil
     import subprocess
     import tempfile
 
     import numpy as np
     mpirun = os.environ.get("MPIRUN", "/opt/conda/bin/mpirun")
-    exes = [_ref_exe(wname, n) for n in REF_ITERS] if wname in REF_WORKLOADS else []
+    exes = [_ref_exe(wname, n, ranks) for n in REF_ITERS] if wname in REF_WORKLOADS else []
     if not exes or not os.path.exists(mpirun) or not all(os.path.exists(e) for e in exes):
         return None
-    c = _ref_case(wname)
+    c = _ref_case(wname, ranks)
     X, Y, Z = c["X"], c["Y"], c["Z"]
     ft = np.zeros((X, Y, Z + 2, 19))
     ft[:, :, 1:Z + 1, :] = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
@@ -197,8 +208,9 @@ def reference_baseline(wname):
             with open(os.path.join(wd, "force.0001"), "wb") as fh:
                 fh.write(np.float64(0.0).tobytes())
             t0 = time.perf_counter()
-            p = subprocess.run([mpirun, "-np", "1", exe, "-p"], cwd=wd, capture_output=True, timeout=600)
+            p = subprocess.run([mpirun, "-np", str(ranks), exe, "-p"], cwd=wd, capture_output=True, timeout=600)
             secs.append(time.perf_counter() - t0)
+            shutil.rmtree(wd, ignore_errors=True)  # the restart and the step-0 outputs
             if p.returncode != 0:
                 print(f"# reference baseline run failed (rc {p.returncode})", file=sys.stderr, flush=True)
                 return None
@@ -207,10 +219,11 @@ def reference_baseline(wname):
     per_step = (secs[1] - secs[0]) / (REF_ITERS[1] - REF_ITERS[0])
     if per_step <= 0:
         return None
-    return {"value": X * Y * Z / per_step / 1e6, "unit": "MLUPS", "cores": 1, "kind": "reference",
-            "sample": f"the reference solver (DNS-solver/src, built from its sources, 1 MPI rank) on a {X}x{Y}x{Z} "
-                      f"slab of the workload, constant flow rate, from rest: ({REF_ITERS[1]} - {REF_ITERS[0]}) steps "
-                      f"between two runs of {secs[0]:.2f} s and {secs[1]:.2f} s"}
+    return {"value": X * Y * Z / per_step / 1e6, "unit": "MLUPS", "cores": ranks, "kind": "reference",
+            "sample": f"the reference solver (DNS-solver/src, built from its sources, {ranks} MPI rank(s)"
+                      f"{', x-split' if ranks > 1 else ''}) on a {X}x{Y}x{Z} slab of the workload, constant flow "
+                      f"rate, from rest: ({REF_ITERS[1]} - {REF_ITERS[0]}) steps between two runs of {secs[0]:.2f} s "
+                      f"and {secs[1]:.2f} s"}
 
 
 def measured_traffic(wname):
@@ -471,9 +484,10 @@ def main():
         sys.exit(2)
     # the CPU baselines run before this process touches the GPU (the
     # reference runs as a child process)
-    cpu = cpu_ref = None
+    cpu = cpu_ref = cpu_mpi = None
     if not args.no_cpu and args.workload in WORKLOADS and not args.z_sweep:
         cpu_ref = reference_baseline(args.workload)
+        cpu_mpi = reference_baseline(args.workload, ranks=REF_MPI_RANKS)
         cpu = cpu_baseline(args.workload)
     dnsgpu.load()
     if args.z_sweep:
@@ -543,6 +557,8 @@ def main():
             out["cpu_baseline_port"] = cpu
     elif cpu:
         out["cpu_baseline"] = cpu
+    if cpu_mpi:
+        out["cpu_baseline_mpi"] = cpu_mpi
     print(json.dumps(out))
 
 

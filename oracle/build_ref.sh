@@ -14,11 +14,14 @@
 #   ITER = number passed to `#define ITER` (the loop runs ITER+1 steps,
 #          dns-const-q-xycut.c:235-237,485)
 #   PSLIP = literal word PSLIP to enable slip posts (slip_posts_init.c)
+# REF_XDIM (environment, default 1): MPI ranks along x (definitions.h:58,
+# the rank count the binary must be started with; XDV a multiple of it)
 set -euo pipefail
 REF=${DNS_REFERENCE_SRC:-/root/reference/DNS-solver/src}
 HERE=$(cd "$(dirname "$0")" && pwd)
 NAME=$1; XDV=$2; YDV=$3; ZDV=$4; REBS=$5; UB=$6; CFL=$7; TSTR=$8; TEND=$9; ITER=${10}
 SLIP=${11:-}
+XDIM=${REF_XDIM:-1}
 OUT=$HERE/_ref/$NAME
 if [ ! -d "$REF" ]; then echo "reference sources not found at $REF" >&2; exit 3; fi
 mkdir -p "$OUT"
@@ -31,6 +34,7 @@ sed -e "s/^#define Rebs .*/#define Rebs $REBS/" \
     -e "s/^#define TEND .*/#define TEND $TEND/" \
     -e "s/^#define CFL .*/#define CFL $CFL/" \
     -e "s/^\/\/#define ITER .*/#define ITER $ITER/" \
+    -e "s/^#define XDIM .*/#define XDIM $XDIM/" \
     "$REF/definitions.h" > "$OUT/definitions.h.tmp"
 if [ "$SLIP" = "PSLIP" ]; then
   sed -i -e "s/^\/\/#define PSLIP/#define PSLIP/" "$OUT/definitions.h.tmp"
