@@ -70,6 +70,9 @@ def test_save_load_roundtrip_bitwise():
     ((16, 8, 10), "posts"),
     ((16, 12, 70), "stripes"),
     ((8, 8, 130), "posts"),
+    ((2, 2, 4), "smooth"),      # the smallest grid the library takes
+    ((3, 5, 65), "stripes"),    # odd x/y extents, a one-row last chunk
+    ((4, 4, 512), "posts"),     # the largest zdv (8 chunks, 3 waves per block)
 ])
 def test_const_force_fp64_bitwise(shape, walls):
     X, Y, Z = shape
@@ -239,7 +242,7 @@ def make_riblet_pair(X, Y, Z, forcing, spacing, height, F=0.0, precision=dnsgpu.
     return ch, o, s
 
 
-@pytest.mark.parametrize("shape,spacing,height", [((8, 16, 24), 8, 4), ((4, 24, 140), 12, 6)])
+@pytest.mark.parametrize("shape,spacing,height", [((8, 16, 24), 8, 4), ((4, 24, 140), 12, 6), ((4, 16, 512), 8, 5)])
 def test_riblets_const_force_fp64_bitwise(shape, spacing, height):
     X, Y, Z = shape
     print(f"case: riblets const-F {shape} s={spacing} h={height}", flush=True)
@@ -282,13 +285,14 @@ def test_riblets_const_q_fp64_and_outputs():
 REDUCED_BARS = {dnsgpu.FP32: (1e-5, 1e-3), dnsgpu.FP16: (1e-4, 3e-3)}
 
 
-@pytest.mark.parametrize("precision", [dnsgpu.FP32, dnsgpu.FP16])
-@pytest.mark.parametrize("texture", ["riblets", "blocks"])
+@pytest.mark.parametrize("precision,texture",
+                         [(p, t) for t in ("riblets", "blocks") for p in (dnsgpu.FP32, dnsgpu.FP16)]
+                         + [(dnsgpu.FP32, "riblets_tall")])
 def test_solid_bands_reduced_storage_against_fp64(texture, precision):
-    X, Y, Z = (8, 16, 32) if texture == "riblets" else (12, 16, 32)
+    X, Y, Z = {"riblets": (8, 16, 32), "blocks": (12, 16, 32), "riblets_tall": (4, 16, 512)}[texture]
     print(f"case: {texture} precision {precision} vs fp64", flush=True)
     kw = dict(blocks=(4, 4, 2, 2, 3)) if texture == "blocks" else {}
-    sp, h = (8, 4) if texture == "riblets" else (0, 0)
+    sp, h = {"riblets": (8, 4), "blocks": (0, 0), "riblets_tall": (8, 5)}[texture]
     a, _, s = make_riblet_pair(X, Y, Z, dnsgpu.CONST_Q, sp, h, **kw)
     b, _, _ = make_riblet_pair(X, Y, Z, dnsgpu.CONST_Q, sp, h, precision=precision, **kw)
     seeded(a)
@@ -302,6 +306,38 @@ def test_solid_bands_reduced_storage_against_fp64(texture, precision):
     assert np.abs(la["force"] - lb["force"]).max() <= force_bar * np.abs(la["force"]).max()
     # solid nodes stay zero in the checkpoint layout
     assert np.all(fb[s != 0] == 0.0)
+
+
+def test_fp16_tall_channel_bands_add_no_error():
+    """FP16 storage at the largest zdv (512): the error against FP64 grows with
+    the step count in the bulk of the channel, with or without riblets (2.0e-4
+    in the populations after 300 steps here, at k = 490 far from any band):
+    per-step changes below half an FP16 ulp of g 2^16 are lost while the tall
+    channel's flow evolves slowly.  The band path must add nothing to it."""
+    X, Y, Z = 4, 16, 512
+    print("case: FP16 tall channel, riblets vs smooth", flush=True)
+
+    def run(precision, bands):
+        ch = dnsgpu.Channel(X, Y, Z, Rebs=60.0, ub=0.05, CFL=0.25, TSTR=5, forcing=dnsgpu.CONST_Q,
+                            precision=precision)
+        if bands:
+            hb = fields.riblet_bands(X, Y, 8, 5)
+            ch.set_solid_bands(hb, hb)
+        seeded(ch)
+        force = ch.step(300)["force"]
+        f = ch.save_aos()[0]
+        ch.close()
+        return f, force
+
+    err = {}
+    for bands in (True, False):
+        fa, ga = run(dnsgpu.FP64, bands)
+        fb, gb = run(dnsgpu.FP16, bands)
+        err[bands] = (float(np.abs(fa - fb).max()), float(np.abs(ga - gb).max() / np.abs(ga).max()))
+    print("FP16 vs FP64 (population, relative force):", err, flush=True)
+    assert err[False][0] <= 5e-4 and err[True][0] <= 5e-4, err
+    assert err[True][0] <= 1.2 * err[False][0] + 1e-5, err
+    assert err[False][1] <= 2e-2 and err[True][1] <= 2e-2, err
 
 
 @pytest.mark.parametrize("blocks", [(4, 4, 2, 2, 3), (6, 8, 3, 2, 5)])
