@@ -121,19 +121,43 @@ __device__ __forceinline__ void collide_fp64(double (&f)[Q], double ueq0, double
     });
 }
 
-// The same update on shifted populations g = f - w in FP32 (FMA allowed):
+// Coefficients of the FP32 collision, per weight class (rest, axis,
+// diagonal), chosen as single-precision numbers whose sums are exact in real
+// arithmetic, so that the FP32 collision conserves mass and momentum up to
+// unbiased rounding (the single-precision weights fl(1/18), fl(1/36) give
+// 18 fl(1/18) = 1 + 7.5e-9: a systematic gain of 1.5e-8 of the momentum per
+// step, about 0.5% of the body force of a Re_tau 180-395 channel):
+//   W (the weights):                 w0 + 6 wa + 12 wd = 1,  wa = 1/18 (1 - 6e-8), wd = wa/2
+//   C (3 w, the linear term):        2 ca + 8 cd = 1,        cd = fl(1/12)
+//   D (4.5 w, the quadratic term):   2 da + 8 dd = 3/2,      da = 1/4, dd = 1/8 (exact)
+//   P (9 w, the force's e.u term):   2 pa + 8 pd = 3,        pa = 1/2, pd = 1/4 (exact)
+// A coefficient is never multiplied into a node-independent factor (omega,
+// or rho = 1 exactly near rest) before the node's own terms: such a product
+// rounds the same way at every node and biases the sums the exact
+// coefficients keep.
+constexpr float kFW[3] = {0x1.555558p-2f, 0x1.c71c70p-5f, 0x1.c71c70p-6f};
+constexpr float kFC[3] = {0.0f, 0x1.555554p-3f, 0x1.555556p-4f};
+constexpr float kFD[3] = {0.0f, 0.25f, 0.125f};
+constexpr float kFP[3] = {0.0f, 0.5f, 0.25f};
+
+// The BGK update with the reference's force term on shifted populations
+// g = f - w in FP32 (FMA allowed):
 //   drho = sum g, rho = 1 + drho, j = sum e g, u = j/rho
-//   g' = g + Fi - omega*(g - (feq - w)),  feq - w = w*(drho + rho*(3eu + 4.5eu^2 - 1.5u^2))
-// evaluated a pair of opposite directions (a, b = opp(a)) at a time, which
-// share w, |e.u| and e_x u_x:
+//   g' = (1 - omega) g + omega (feq - w) + Fi
+//   feq - w = W drho + rho (C e.u + D (e.u)^2 - W 1.5 u^2)
+//   Fi      = G (-3 W u_x + e_x (C + P e.u))
+// (with W, C, D, P = w, 3w, 4.5w, 9w this is xcomputations.c:330-341), evaluated
+// a pair of opposite directions (a, b = opp(a)) at a time, which share the
+// class coefficients, (e.u)^2 and e_x e.u:
 //   moments from g_a + g_b and g_a - g_b;
-//   q = 4.5eu^2 - 1.5u^2, poly_a,b = q +- 3eu;
-//   omega (feq - w) = K1 poly + K2 with K1 = omega w rho, K2 = omega w drho;
-//   Fi_a,b = t +- 3 e_x w G, t = w G (9 e_x eu - 3 u_x)  (Fi = w G (-3 u_x) when e_x = 0);
-//   g'_a = (1 - omega) g_a + K1 poly_a + K2 + Fi_a.
+//   per class: R = W (omega drho - 3 G u_x), R+- = R +- C G, V = -W 1.5 u^2;
+//   m = D (e.u)^2 + V;
+//   e_x = 0:  g'_a,b = (1 - omega) g_a,b + omega rho (m +- C e.u) + R;
+//   e_x = s:  g'_a = (1 - omega) g_a + omega rho (m + C e.u) + (R_s + s P G e.u),
+//             g'_b = (1 - omega) g_b + omega rho (m - C e.u) + (R_-s + s P G e.u).
 // S: the populations are held multiplied by S (a power of two): S = 1 for
 // FP32 storage, 2^16 for the FP16 storage's scaled values; the moments are
-// unscaled once and the weights carry S, so no per-population rescaling.
+// unscaled once and the coefficients carry S, so no per-population rescaling.
 template <int LOG2S = 0>
 __device__ __forceinline__ void collide_fp32(float (&g)[Q], float omega, float Gx, float& drho, float& u0,
                                              float& u1, float& u2, float& jx)
@@ -164,48 +188,52 @@ __device__ __forceinline__ void collide_fp32(float (&g)[Q], float omega, float G
     }
     drho = sr;
     jx = sx;
-    const float rho = 1.0f + sr;
-    const float rinv = 1.0f / rho;
-    u0 = sx * rinv;
-    u1 = sy * rinv;
-    u2 = sz * rinv;
+    // u = j / (1 + drho) as j - j q with q = drho / (1 + drho), and
+    // omega rho = omega + omega drho: near rest, 1 + drho rounds to one of a
+    // few numbers next to 1 and the rounding of 1/rho would bias u by drho^2
+    const float rinv = 1.0f / (1.0f + sr);
+    const float q = sr * rinv;
+    u0 = __builtin_fmaf(-sx, q, sx);
+    u1 = __builtin_fmaf(-sy, q, sy);
+    u2 = __builtin_fmaf(-sz, q, sz);
     const float v15 = 1.5f * __builtin_fmaf(u0, u0, __builtin_fmaf(u1, u1, u2 * u2));
     const float om1 = 1.0f - omega;
-    const float m3u0 = -3.0f * u0;
-    // per weight class k (rest 1/3, axis 1/18, diagonal 1/36, times S)
-    constexpr float Wk[3] = {(float)(1. / 3.) * S, (float)(1. / 18.) * S, (float)(1. / 36.) * S};
-    float K1[3], K2C[3], K2p[3], K2m[3], Ck[3], Pk[3];
+    const float orho = __builtin_fmaf(omega, sr, omega);
+    // omega drho - 3 G u_x: a node's own value, multiplied by each W below
+    const float t0 = __builtin_fmaf(-3.0f * Gx, u0, omega * sr);
+    float R[3], Rp[3], Rm[3], V[3], pG[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-        const float ow = omega * Wk[k], wG = Wk[k] * Gx;
-        K1[k] = ow * rho;
-        const float K2 = ow * sr;
-        Ck[k] = wG * m3u0;
-        K2C[k] = K2 + Ck[k];
-        K2p[k] = __builtin_fmaf(3.0f, wG, K2);
-        K2m[k] = __builtin_fmaf(-3.0f, wG, K2);
-        Pk[k] = 9.0f * wG;
+        R[k] = (kFW[k] * S) * t0;
+        V[k] = (kFW[k] * S) * -v15;
+        const float cG = (kFC[k] * S) * Gx;
+        Rp[k] = R[k] + cG;
+        Rm[k] = R[k] - cG;
+        pG[k] = (kFP[k] * S) * Gx;  // exact: P S is a power of two
     }
-    g[0] = __builtin_fmaf(om1, g[0], __builtin_fmaf(K1[0], -v15, K2C[0]));
+    g[0] = __builtin_fmaf(om1, g[0], __builtin_fmaf(orho, V[0], R[0]));
     static_for<1, Q>([&](auto ac) {
         constexpr int a = decltype(ac)::value;
         if constexpr (a < OPP[a]) {
             constexpr int b = OPP[a];
             constexpr int k = (EX[a] != 0) + (EY[a] != 0) + (EZ[a] != 0);  // 1 axis, 2 diagonal
+            constexpr float C = kFC[k] * S, D = kFD[k] * S;
             const float eu = edotf<a>(u0, u1, u2);
-            const float q = __builtin_fmaf(4.5f * eu, eu, -v15);
-            const float pa = __builtin_fmaf(3.0f, eu, q), pb = __builtin_fmaf(-3.0f, eu, q);
-            float ia, ib;
+            const float m = __builtin_fmaf(D, eu * eu, V[k]);
+            const float ea = __builtin_fmaf(C, eu, m), eb = __builtin_fmaf(-C, eu, m);
+            float ra, rb;
             if constexpr (EX[a] == 0) {
-                ia = __builtin_fmaf(K1[k], pa, K2C[k]);
-                ib = __builtin_fmaf(K1[k], pb, K2C[k]);
+                ra = R[k];
+                rb = R[k];
+            } else if constexpr (EX[a] > 0) {
+                ra = __builtin_fmaf(pG[k], eu, Rp[k]);
+                rb = __builtin_fmaf(pG[k], eu, Rm[k]);
             } else {
-                const float t = __builtin_fmaf(EX[a] > 0 ? Pk[k] : -Pk[k], eu, Ck[k]);
-                ia = __builtin_fmaf(K1[k], pa, (EX[a] > 0 ? K2p[k] : K2m[k]) + t);
-                ib = __builtin_fmaf(K1[k], pb, (EX[a] > 0 ? K2m[k] : K2p[k]) + t);
+                ra = __builtin_fmaf(-pG[k], eu, Rm[k]);
+                rb = __builtin_fmaf(-pG[k], eu, Rp[k]);
             }
-            g[a] = __builtin_fmaf(om1, g[a], ia);
-            g[b] = __builtin_fmaf(om1, g[b], ib);
+            g[a] = __builtin_fmaf(om1, g[a], __builtin_fmaf(orho, ea, ra));
+            g[b] = __builtin_fmaf(om1, g[b], __builtin_fmaf(orho, eb, rb));
         }
     });
 }

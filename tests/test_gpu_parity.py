@@ -394,3 +394,28 @@ def test_grid_convergence_of_flow_over_rectangular_ribs():
     order = np.log2(e1 / e2)
     assert 0.9 <= order <= 2.5, (R, order)
     assert abs(R[2] - 0.7058) < 2e-3, R
+
+
+# relative change of the total x-momentum over 1000 steps with slip walls and
+# no force (momentum is a collision and streaming invariant there): FP64
+# sums in the reference's order; FP32 storage: the collision's coefficient
+# sums are exact, so only unbiased rounding is left (single-precision
+# weights fl(1/18), fl(1/36) would gain 1.5e-8 of the momentum per step,
+# 1.5e-5 here); FP16 storage: its rounding of each stored population
+MOMENTUM_BARS = {dnsgpu.FP64: 1e-12, dnsgpu.FP32: 1e-7, dnsgpu.FP16: 3e-5}
+
+
+@pytest.mark.parametrize("precision", [dnsgpu.FP64, dnsgpu.FP32, dnsgpu.FP16])
+def test_momentum_conserved_with_slip_walls(precision):
+    X, Y, Z = 16, 16, 32
+    print(f"case: slip walls, no force, precision {precision}", flush=True)
+    # nu = 0.005: the seeded noise lives through the run, so nodes differ
+    ch = dnsgpu.Channel(X, Y, Z, ub=0.05, CFL=0.25, forcing=dnsgpu.CONST_F, force=0.0, precision=precision,
+                        omega=1.0 / (3 * 0.005 + 0.5))
+    slip = np.full((X, Y), -2, dtype=np.int8)
+    ch.set_walls(slip, slip)
+    ch.init_field(dnsgpu.INIT_POISEUILLE, uc=0.08, amp=0.1, seed=4)
+    j = ch.step(1000)["massrate"]
+    drift = float(np.abs(j - j[0]).max() / abs(j[0]))
+    print(f"relative momentum drift {drift:.3e}", flush=True)
+    assert drift <= MOMENTUM_BARS[precision]
