@@ -6,11 +6,19 @@
 // the practical ceiling of the step kernel's traffic pattern.
 //
 // usage: copybw [GiB=4] [reps=5] [all]
-//   all: also a copy with four 16-byte loads in flight per thread, a copy
-//        by contiguous per-block chunks, a read-only and a write-only stream
+//   all: also the copy variants below, a read-only and a write-only stream:
+//     x4       four 16-byte loads in flight per thread
+//     chunks   each block copies one contiguous chunk
+//     wave64k  each wave copies contiguous 64 KiB chunks (dwordx4), chunks
+//              grid-strided over the waves
+//     nt       non-temporal loads and stores (__builtin_nontemporal_*)
+//     ntstore  plain loads, non-temporal stores
+//     b1024    the grid-stride copy with 1024-thread blocks
+//     memcpy   hipMemcpyAsync device to device
 #include <hip/hip_runtime.h>
 #include <stdio.h>
 #include <stdlib.h>
+#include <string.h>
 
 typedef float f4 __attribute__((ext_vector_type(4)));
 
@@ -24,6 +32,12 @@ typedef float f4 __attribute__((ext_vector_type(4)));
     } while (0)
 
 __global__ __launch_bounds__(256) void copy_f4(const f4* __restrict__ a, f4* __restrict__ b, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        b[i] = a[i];
+}
+
+__global__ __launch_bounds__(1024) void copy_f4_b1024(const f4* __restrict__ a, f4* __restrict__ b, size_t n)
 {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         b[i] = a[i];
@@ -54,6 +68,36 @@ __global__ __launch_bounds__(256) void copy_f4_chunks(const f4* __restrict__ a, 
     for (size_t i = lo + threadIdx.x; i < hi; i += blockDim.x) b[i] = a[i];
 }
 
+// each wave copies whole 64 KiB chunks (4096 float4), chunk c by wave
+// c mod (waves in the grid); the last chunk may be partial (i < n checked)
+__global__ __launch_bounds__(256) void copy_f4_wave64k(const f4* __restrict__ a, f4* __restrict__ b, size_t n)
+{
+    const int lane = threadIdx.x & 63;
+    const size_t wave = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6;
+    const size_t nwaves = ((size_t)gridDim.x * blockDim.x) >> 6;
+    const size_t nchunks = (n + 4095) / 4096;
+    for (size_t c = wave; c < nchunks; c += nwaves) {
+        const size_t base = c * 4096;
+#pragma unroll 4
+        for (int k = 0; k < 64; k++) {
+            const size_t i = base + (size_t)k * 64 + lane;
+            if (i < n) b[i] = a[i];
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void copy_f4_nt(const f4* __restrict__ a, f4* __restrict__ b, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(a + i), b + i);
+}
+
+__global__ __launch_bounds__(256) void copy_f4_ntstore(const f4* __restrict__ a, f4* __restrict__ b, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        __builtin_nontemporal_store(a[i], b + i);
+}
+
 // read-only stream (the sum is stored only if it equals a value the zeroed input never gives)
 __global__ __launch_bounds__(256) void read_f4(const f4* __restrict__ a, f4* __restrict__ out, size_t n)
 {
@@ -68,11 +112,31 @@ __global__ __launch_bounds__(256) void write_f4(f4* __restrict__ b, size_t n)
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) b[i] = z;
 }
 
+enum Variant { X4, CHUNKS, WAVE64K, NT, NTSTORE, B1024, MEMCPY, READ, WRITE, NVARIANT };
+static const char* kNames[NVARIANT] = {"copy, 4 loads in flight", "copy, block chunks", "copy, wave 64 KiB chunks",
+                                       "copy, non-temporal",      "copy, nt stores",    "copy, 1024-thr blocks (g/4)",
+                                       "hipMemcpyAsync D2D",      "read only",          "write only"};
+
+static void launch(int k, int g, const f4* a, f4* b, size_t n, size_t bytes)
+{
+    switch (k) {
+    case X4: hipLaunchKernelGGL(copy_f4_x4, dim3(g), dim3(256), 0, 0, a, b, n); break;
+    case CHUNKS: hipLaunchKernelGGL(copy_f4_chunks, dim3(g), dim3(256), 0, 0, a, b, n); break;
+    case WAVE64K: hipLaunchKernelGGL(copy_f4_wave64k, dim3(g), dim3(256), 0, 0, a, b, n); break;
+    case NT: hipLaunchKernelGGL(copy_f4_nt, dim3(g), dim3(256), 0, 0, a, b, n); break;
+    case NTSTORE: hipLaunchKernelGGL(copy_f4_ntstore, dim3(g), dim3(256), 0, 0, a, b, n); break;
+    case B1024: hipLaunchKernelGGL(copy_f4_b1024, dim3(g / 4 > 0 ? g / 4 : 1), dim3(1024), 0, 0, a, b, n); break;
+    case MEMCPY: (void)hipMemcpyAsync(b, a, bytes, hipMemcpyDeviceToDevice, 0); break;
+    case READ: hipLaunchKernelGGL(read_f4, dim3(g), dim3(256), 0, 0, a, b, n); break;
+    case WRITE: hipLaunchKernelGGL(write_f4, dim3(g), dim3(256), 0, 0, b, n); break;
+    }
+}
+
 int main(int argc, char** argv)
 {
     const double gib = argc > 1 ? atof(argv[1]) : 4.0;
     const int reps = argc > 2 ? atoi(argv[2]) : 5;
-    const size_t bytes = (size_t)(gib * (1ull << 30)) & ~(size_t)15;
+    const size_t bytes = (size_t)(gib * (1ull << 30)) & ~(size_t)65535;
     const size_t n = bytes / 16;
     void *a, *b;
     CHK(hipMalloc(&a, bytes));
@@ -99,17 +163,14 @@ int main(int argc, char** argv)
         printf("grid %5d x 256: %8.1f GB/s  (%.3f ms, best of %d)\n", g, 2.0 * bytes / best / 1e6, best, reps - 1);
     }
     if (argc > 3) {
-        const char* names[] = {"copy, 4 loads in flight", "copy, block chunks", "read only", "write only"};
         const int grids2[] = {512, 1024, 2048, 4096, 16384};
-        for (int k = 0; k < 4; k++) {
+        for (int k = 0; k < NVARIANT; k++) {
             for (int g : grids2) {
+                if (k == MEMCPY && g != grids2[0]) break;  // no grid to choose
                 float best = 1e30f;
                 for (int r = 0; r < reps; r++) {
                     CHK(hipEventRecord(e0));
-                    if (k == 0) hipLaunchKernelGGL(copy_f4_x4, dim3(g), dim3(256), 0, 0, (const f4*)a, (f4*)b, n);
-                    if (k == 1) hipLaunchKernelGGL(copy_f4_chunks, dim3(g), dim3(256), 0, 0, (const f4*)a, (f4*)b, n);
-                    if (k == 2) hipLaunchKernelGGL(read_f4, dim3(g), dim3(256), 0, 0, (const f4*)a, (f4*)b, n);
-                    if (k == 3) hipLaunchKernelGGL(write_f4, dim3(g), dim3(256), 0, 0, (f4*)b, n);
+                    launch(k, g, (const f4*)a, (f4*)b, n, bytes);
                     CHK(hipGetLastError());
                     CHK(hipEventRecord(e1));
                     CHK(hipEventSynchronize(e1));
@@ -117,8 +178,12 @@ int main(int argc, char** argv)
                     CHK(hipEventElapsedTime(&ms, e0, e1));
                     if (r > 0 && ms < best) best = ms;
                 }
-                const double moved = (k <= 1 ? 2.0 : 1.0) * bytes;
-                printf("%-24s grid %5d x 256: %8.1f GB/s  (best of %d)\n", names[k], g, moved / best / 1e6, reps - 1);
+                const double moved = (k == READ || k == WRITE ? 1.0 : 2.0) * bytes;
+                if (k == MEMCPY)
+                    printf("%-26s            : %8.1f GB/s  (best of %d)\n", kNames[k], moved / best / 1e6, reps - 1);
+                else
+                    printf("%-26s grid %5d x 256: %8.1f GB/s  (best of %d)\n", kNames[k], g, moved / best / 1e6,
+                           reps - 1);
             }
         }
     }

@@ -590,7 +590,127 @@ extern "C" int dnsgpu_set_solid_bands(dnsgpu* h, const int16_t* bottom, const in
     HIPCHK(hipStreamSynchronize(h->stream));
     h->has_bands = any ? 1 : 0;
     h->nfluid = total;
+    // the force of the first step and the plane-mean shifts were derived by
+    // the last init or load over every node; derive them again over the
+    // fluid nodes (solid nodes may hold anything, zeros in a checkpoint)
+    int rc = refresh_force(h);
+    if (rc) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
     return 0;
+}
+
+// Tables of the seeded turbulent field (DNSGPU_INIT_TURBULENT), laid out in
+// one host array t: [nm][kModeRec] modes, [mx+1][X] cos and sin of
+// 2 pi i x / X, [nm][4][Y] cos and sin of ky y + p1 and ky y + p2, [3][Z] the
+// log-law mean, the envelope and its derivative.  Every transcendental value
+// is one libm call on an argument formed in the order written here, and the
+// sin and cos of one argument are taken in separate loops (each a plain libm
+// call), so the tables are reproducible by any program that calls the same
+// libm.  Modes: 0 <= i <= mx, -my <= j <= my, (i, j) > (0, 0), with
+// mx = min(nmodes, (X-1)/2) and my = min(nmodes, (Y-1)/2): no two modes alias
+// on the grid and no mode is its own mirror, so the modes are orthogonal over
+// the periodic planes and the rms of the field follows from the tables alone.
+static void turbulent_tables(const dnsgpu* h, const dns_init* spec, std::vector<double>& t, InitArgs& A,
+                             size_t off[5])
+{
+    const int X = h->g.X, Y = h->g.Y, Z = h->g.Z;
+    const double pi = 3.141592653589793, two_pi = 6.283185307179586;
+    const double kinv = 1.0 / 0.41, Bplus = 5.2;
+    const double uc = spec->uc, amp = spec->amp;
+    const double nu = (1. / 3.) * (1. / h->omega - 0.5);
+    const double hh = 0.5 * (double)Z;
+    const int mx = std::min(spec->nmodes, (X - 1) / 2), my = std::min(spec->nmodes, (Y - 1) / 2);
+    struct Mode {
+        int i, j;
+        double sc, a1, a2, p1, p2, kx, ky;
+    };
+    std::vector<Mode> modes;
+    for (int i = 0; i <= mx; i++)
+        for (int j = -my; j <= my; j++) {
+            if (i == 0 && j <= 0) continue;
+            const unsigned long long m = (unsigned long long)((i * 64 + (j + 32)) & 0xFFFF);
+            Mode md;
+            md.i = i;
+            md.j = j;
+            md.sc = 1.0 / (double)(i * i + j * j);
+            md.a1 = urand(spec->seed, m * 8 + 0);
+            md.a2 = urand(spec->seed, m * 8 + 1);
+            md.p1 = pi * urand(spec->seed, m * 8 + 2);
+            md.p2 = pi * urand(spec->seed, m * 8 + 3);
+            md.kx = (two_pi * (double)i) / (double)X;
+            md.ky = (two_pi * (double)j) / (double)Y;
+            modes.push_back(md);
+        }
+    const int nm = (int)modes.size();
+    off[0] = 0;
+    off[1] = off[0] + (size_t)nm * kModeRec;
+    off[2] = off[1] + (size_t)(mx + 1) * X;
+    off[3] = off[2] + (size_t)(mx + 1) * X;
+    off[4] = off[3] + (size_t)nm * 4 * Y;
+    t.assign(off[4] + 3 * (size_t)Z, 0.0);
+    for (int m = 0; m < nm; m++) {
+        double* r = &t[off[0] + (size_t)m * kModeRec];
+        r[0] = (double)modes[m].i;
+        r[1] = modes[m].sc;
+        r[2] = modes[m].a1;
+        r[3] = modes[m].a2;
+        r[4] = modes[m].kx;
+        r[5] = modes[m].ky;
+    }
+    for (int i = 0; i <= mx; i++)
+        for (int x = 0; x < X; x++) t[off[1] + (size_t)i * X + x] = cos((two_pi * (double)((i * x) % X)) / (double)X);
+    for (int i = 0; i <= mx; i++)
+        for (int x = 0; x < X; x++) t[off[2] + (size_t)i * X + x] = sin((two_pi * (double)((i * x) % X)) / (double)X);
+    for (int m = 0; m < nm; m++)
+        for (int q = 0; q < 2; q++)
+            for (int y = 0; y < Y; y++) {
+                const int n = ((modes[m].j * y) % Y + Y) % Y;
+                const double arg = (two_pi * (double)n) / (double)Y + (q ? modes[m].p2 : modes[m].p1);
+                t[off[3] + ((size_t)m * 4 + 2 * q) * Y + y] = cos(arg);
+            }
+    for (int m = 0; m < nm; m++)
+        for (int q = 0; q < 2; q++)
+            for (int y = 0; y < Y; y++) {
+                const int n = ((modes[m].j * y) % Y + Y) % Y;
+                const double arg = (two_pi * (double)n) / (double)Y + (q ? modes[m].p2 : modes[m].p1);
+                t[off[3] + ((size_t)m * 4 + 2 * q + 1) * Y + y] = sin(arg);
+            }
+    double* U = &t[off[4]];
+    double* env = U + Z;
+    double* denv = U + 2 * Z;
+    std::vector<double> sn(Z), cs(Z);
+    for (int z = 0; z < Z; z++) {
+        const double yy = (double)z + 0.5;
+        const double yw = yy < hh ? yy : (double)Z - yy;
+        const double yplus = yw * uc / nu;
+        const double uplus = yplus < 11.0 ? yplus : (log(yplus) * kinv + Bplus);
+        U[z] = uc * uplus;
+    }
+    for (int z = 0; z < Z; z++) sn[z] = sin(pi * ((double)z + 0.5) / (double)Z);
+    for (int z = 0; z < Z; z++) cs[z] = cos(pi * ((double)z + 0.5) / (double)Z);
+    for (int z = 0; z < Z; z++) {
+        env[z] = sn[z] * sn[z];
+        denv[z] = (((2.0 * sn[z]) * cs[z]) * pi) / (double)Z;
+    }
+    // rms of the unscaled fluctuation over the grid: mean over the planes of
+    // sum_m sc^2 (denv^2 (a1^2 + a2^2) + env^2 ((a1 ky)^2 + (a2 kx)^2
+    //   - 2 (a1 ky)(a2 kx) cos(p1 - p2))) / 2
+    std::vector<double> cpp(nm);
+    for (int m = 0; m < nm; m++) cpp[m] = cos(modes[m].p1 - modes[m].p2);
+    double S = 0.0;
+    for (int z = 0; z < Z; z++) {
+        const double d2 = denv[z] * denv[z], e2 = env[z] * env[z];
+        for (int m = 0; m < nm; m++) {
+            const Mode& md = modes[m];
+            const double q1 = md.a1 * md.ky, q2 = md.a2 * md.kx;
+            const double term = (md.sc * md.sc) * (d2 * ((md.a1 * md.a1 + md.a2 * md.a2) * 0.5) +
+                                                   e2 * (((q1 * q1 + q2 * q2) - 2.0 * (q1 * q2) * cpp[m]) * 0.5));
+            S = S + term;
+        }
+    }
+    const double ucl = uc * (log(hh * uc / nu) * kinv + Bplus);
+    A.nm = nm;
+    A.nrm = S > 0.0 ? amp * ucl / sqrt(S / (double)Z) : 0.0;
 }
 
 static int launch_init(dnsgpu* h, int kind, const dns_init* spec)
@@ -605,11 +725,22 @@ static int launch_init(dnsgpu* h, int kind, const dns_init* spec)
         A.uc = spec->uc;
         A.amp = spec->amp;
         A.seed = spec->seed;
-        A.nmodes = spec->nmodes;
     }
-    A.nu = (1. / 3.) * (1. / h->omega - 0.5);
-    A.kappa_inv = 1.0 / 0.41;
-    A.Bplus = 5.2;
+    if (kind == DNSGPU_INIT_TURBULENT) {
+        std::vector<double> t;
+        size_t off[5];
+        turbulent_tables(h, spec, t, A, off);
+        int rc = stage_reserve(h, sizeof(double) * t.size());
+        if (rc) return rc;
+        HIPCHK(hipMemcpyAsync(h->stage, t.data(), sizeof(double) * t.size(), hipMemcpyHostToDevice, h->stream));
+        A.mode = h->stage + off[0];
+        A.cx = h->stage + off[1];
+        A.sx = h->stage + off[2];
+        A.by = h->stage + off[3];
+        A.zt = h->stage + off[4];
+        // the host array t must outlive the copy
+        HIPCHK(hipStreamSynchronize(h->stream));
+    }
     const long long n = (long long)h->g.X * h->g.Y * h->g.Z;
     int grid = (int)std::min<long long>((n + 255) / 256, 65536);
     with_storage(h->c.precision, [&](auto t) {

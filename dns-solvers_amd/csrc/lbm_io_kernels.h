@@ -315,12 +315,19 @@ struct InitArgs {
     int kind;
     double uc, amp;
     unsigned long long seed;
-    int nmodes;
-    double nu;   // viscosity (turbulent profile)
-    double kappa_inv, Bplus;
+    // TURBULENT: tables of the seeded field (turbulent_tables in dnsgpu.hip)
+    int nm;              // Fourier modes
+    double nrm;          // scale of the fluctuation (rms |u'| = amp * centreline velocity)
+    const double* mode;  // [nm][kModeRec]: i, sc, a1, a2, kx, ky
+    const double* cx;    // [mx+1][X] cos(2 pi i x / X)
+    const double* sx;    // [mx+1][X] sin(2 pi i x / X)
+    const double* by;    // [nm][4][Y] cos, sin of (ky y + p1), then of (ky y + p2)
+    const double* zt;    // [3][Z] log-law mean u(z), envelope env(z), d env / dz
 };
 
-__device__ __forceinline__ unsigned long long splitmix64(unsigned long long x)
+constexpr int kModeRec = 6;
+
+__host__ __device__ __forceinline__ unsigned long long splitmix64(unsigned long long x)
 {
     x += 0x9E3779B97F4A7C15ull;
     x = (x ^ (x >> 30)) * 0xBF58476D1CE4E5B9ull;
@@ -329,7 +336,7 @@ __device__ __forceinline__ unsigned long long splitmix64(unsigned long long x)
 }
 
 // Uniform in [-1, 1): 53 random bits, exact arithmetic.
-__device__ __forceinline__ double urand(unsigned long long seed, unsigned long long ctr)
+__host__ __device__ __forceinline__ double urand(unsigned long long seed, unsigned long long ctr)
 {
     const unsigned long long h = splitmix64(seed ^ splitmix64(ctr));
     return (double)(h >> 11) * (1.0 / 9007199254740992.0) * 2.0 - 1.0;
@@ -355,8 +362,10 @@ __device__ __forceinline__ void feq_fp64(double u0, double u1, double u2, double
 //              plus amp*uc*U[-1,1) in each component (counter = node*4+comp).
 //  TURBULENT : mean from the log law u+ = ln(y+)/kappa + B (viscous sublayer
 //              u+ = y+ below y+ = 11), plus the curl of a random vector
-//              potential built from Fourier modes kx, ky <= nmodes with a
-//              sin^2 wall-normal envelope (divergence-free, zero at walls).
+//              potential built from Fourier modes |kx|, |ky| <= nmodes with a
+//              sin^2 wall-normal envelope (divergence-free, zero at walls),
+//              scaled so that its rms magnitude over the grid is amp times
+//              the log law's centreline velocity.
 template <typename T>
 static __global__ __launch_bounds__(256) void init_kernel(InitArgs A)
 {
@@ -380,44 +389,29 @@ static __global__ __launch_bounds__(256) void init_kernel(InitArgs A)
             u1 = A.amp * A.uc * urand(A.seed, node * 4ull + 1ull);
             u2 = A.amp * A.uc * urand(A.seed, node * 4ull + 2ull);
         } else if (A.kind == 2) {
-            const double h = 0.5 * (double)Z;
-            const double yy = (double)z + 0.5;
-            const double yw = yy < h ? yy : (double)Z - yy;
-            const double yplus = yw * A.uc / A.nu;
-            const double uplus = yplus < 11.0 ? yplus : (log(yplus) * A.kappa_inv + A.Bplus);
-            u0 = A.uc * uplus;
-            const double ucl = A.uc * (log(h * A.uc / A.nu) * A.kappa_inv + A.Bplus);
-            // vector potential psi = (0, 0, 0) + sum over modes; u' = curl(psi)
-            // psi_x = env(z) * a1 cos(kx x + ky y + p1), psi_y = env(z) * a2 cos(...),
-            // with env = sin^2(pi (z+1/2)/Z); psi_z = 0 so that u'_z = d psi_y/dx - d psi_x/dy.
-            const double pz = 3.141592653589793 * yy / (double)Z;
-            const double env = sin(pz) * sin(pz);
-            const double denv = 2.0 * sin(pz) * cos(pz) * 3.141592653589793 / (double)Z;
+            // u = U(z) e_x + nrm curl(psi), psi = sum over modes m of
+            //   sc_m env(z) (a1 cos(ph1), a2 cos(ph2), 0),  ph = kx x + ky y + p,
+            // so u'_x = -d psi_y/dz, u'_y = d psi_x/dz, u'_z = d psi_y/dx - d psi_x/dy.
+            // The phases come from the x and y tables by the angle-sum rule;
+            // only IEEE products and sums here (-ffp-contract=off), in this order.
+            const double U = A.zt[z], env = A.zt[Z + z], denv = A.zt[2 * Z + z];
             double ux = 0., uy = 0., uz = 0.;
-            for (int i = 0; i <= A.nmodes; i++)
-                for (int j = -A.nmodes; j <= A.nmodes; j++) {
-                    if (i == 0 && j <= 0) continue;
-                    const unsigned long long m = (unsigned long long)((i * 64 + (j + 32)) & 0xFFFF);
-                    const double a1 = urand(A.seed, m * 8 + 0), a2 = urand(A.seed, m * 8 + 1);
-                    const double p1 = 3.141592653589793 * urand(A.seed, m * 8 + 2);
-                    const double p2 = 3.141592653589793 * urand(A.seed, m * 8 + 3);
-                    const double kx = 2.0 * 3.141592653589793 * i / (double)X;
-                    const double ky = 2.0 * 3.141592653589793 * j / (double)Y;
-                    const double ph1 = kx * x + ky * y + p1, ph2 = kx * x + ky * y + p2;
-                    const double sc = 1.0 / (double)(i * i + j * j);
-                    // psi_x = env a1 cos(ph1), psi_y = env a2 cos(ph2), psi_z = 0
-                    // u'_x = d psi_z/dy - d psi_y/dz = -denv a2 cos(ph2)
-                    // u'_y = d psi_x/dz - d psi_z/dx =  denv a1 cos(ph1)
-                    // u'_z = d psi_y/dx - d psi_x/dy = env(-a2 kx sin(ph2) + a1 ky sin(ph1))
-                    ux += sc * (-denv * a2 * cos(ph2));
-                    uy += sc * (denv * a1 * cos(ph1));
-                    uz += sc * env * (-a2 * kx * sin(ph2) + a1 * ky * sin(ph1));
-                }
-            // normalise the fluctuation amplitude to amp * centreline velocity
-            const double nrm = A.amp * ucl * (double)Z / 3.0;
-            u0 += nrm * ux;
-            u1 = nrm * uy;
-            u2 = nrm * uz;
+            for (int m = 0; m < A.nm; m++) {
+                const double* md = A.mode + (size_t)m * kModeRec;
+                const int i = (int)md[0];
+                const double sc = md[1], a1 = md[2], a2 = md[3], kx = md[4], ky = md[5];
+                const double cA = A.cx[(size_t)i * X + x], sA = A.sx[(size_t)i * X + x];
+                const double* b = A.by + (size_t)m * 4 * Y;
+                const double cb1 = b[y], sb1 = b[Y + y], cb2 = b[2 * Y + y], sb2 = b[3 * Y + y];
+                const double c1 = cA * cb1 - sA * sb1, s1 = sA * cb1 + cA * sb1;
+                const double c2 = cA * cb2 - sA * sb2, s2 = sA * cb2 + cA * sb2;
+                ux = ux + sc * (-((denv * a2) * c2));
+                uy = uy + sc * ((denv * a1) * c1);
+                uz = uz + sc * (env * ((a1 * ky) * s1 - (a2 * kx) * s2));
+            }
+            u0 = U + A.nrm * ux;
+            u1 = A.nrm * uy;
+            u2 = A.nrm * uz;
         }
         double f[Q];
         if (A.kind == 0) {

@@ -260,6 +260,13 @@ class Channel:
             f[x0:x0 + nx] = b
         return ft, f
 
+    def save_slab(self, x0, nx):
+        """Both checkpoint halves of the x-planes [x0, x0 + nx)."""
+        a = np.zeros(self.aos_shape(nx))
+        b = np.zeros(self.aos_shape(nx))
+        _chk(load().dnsgpu_save_aos(self.h, x0, nx, _dp(a), _dp(b)))
+        return a, b
+
     def force(self):
         g = ctypes.c_double()
         _chk(load().dnsgpu_get_force(self.h, ctypes.byref(g)))

@@ -96,12 +96,20 @@ enum dnsgpu_init_kind {
     DNSGPU_INIT_TURBULENT = 2   /* feq(1, u): log-law mean + divergence-free Fourier modes */
 };
 
+/* TURBULENT: u = U(z) e_x + u', U the log law u_tau (ln(y+)/0.41 + 5.2)
+ * (u_tau y+ below y+ = 11) from the nearer wall, u' the curl of a vector
+ * potential sum_m sc_m env(z) (a1 cos(ph1), a2 cos(ph2), 0) with
+ * env = sin^2(pi y / zdv), ph = 2 pi (i x / xdv + j y / ydv) + p, seeded
+ * a1, a2 in [-1, 1), p in [-pi, pi), sc_m = 1 / (i^2 + j^2), over the modes
+ * 0 <= i <= min(nmodes, (xdv-1)/2), |j| <= min(nmodes, (ydv-1)/2),
+ * (i, j) > (0, 0): divergence-free and zero at the walls. */
 typedef struct dns_init {
     int kind;      /* enum dnsgpu_init_kind */
     double uc;     /* centreline velocity (POISEUILLE) / friction velocity u_tau (TURBULENT) */
-    double amp;    /* noise amplitude as a fraction of uc (POISEUILLE) or of the centreline (TURBULENT) */
+    double amp;    /* POISEUILLE: noise amplitude as a fraction of uc; TURBULENT: rms of |u'| over the
+                      grid as a fraction of the log law's centreline velocity U(zdv/2) */
     uint64_t seed; /* counter-based RNG seed */
-    int nmodes;    /* TURBULENT: Fourier modes per periodic direction (kx, ky <= nmodes) */
+    int nmodes;    /* TURBULENT: Fourier modes per periodic direction, 1..31 */
 } dns_init;
 
 typedef struct dnsgpu dnsgpu;

@@ -154,3 +154,110 @@ def band_mask(X, Y, Z, hb, ht, bottom=None, top=None):
     s[(k >= 1) & (k <= hb)] = -1
     s[(k <= Z) & (k >= Z + 1 - ht)] = -1
     return s
+
+
+# --------------------------------------------------------------------------
+# Replica of the library's DNSGPU_INIT_TURBULENT generator (the table
+# builder turbulent_tables in dns-solvers_amd/csrc/dnsgpu.hip and the
+# TURBULENT branch of init_kernel in csrc/lbm_io_kernels.h).  The tables are
+# built with Python's math module, which calls the C library's cos, sin and
+# log on arguments formed in the library's order; the per-node sums are the
+# library's IEEE products and sums in its order, so the result is bit-exact.
+
+import math as _math
+
+_PI = 3.141592653589793
+_TWO_PI = 6.283185307179586
+
+
+def turbulent_tables(X, Y, Z, omega, uc, amp, seed, nmodes):
+    """The generator's modes, x/y/z tables and scale: a dict with 'modes'
+    (list of (i, j, sc, a1, a2, p1, p2, kx, ky)), 'cx', 'sx' [mx+1][X], 'by'
+    [nm][4][Y], 'U', 'env', 'denv' [Z], 'nrm', 'ucl', 'nu'."""
+    kinv, Bplus = 1.0 / 0.41, 5.2
+    nu = (1. / 3.) * (1. / omega - 0.5)
+    hh = 0.5 * float(Z)
+    mx, my = min(nmodes, (X - 1) // 2), min(nmodes, (Y - 1) // 2)
+    modes = []
+    for i in range(mx + 1):
+        for j in range(-my, my + 1):
+            if i == 0 and j <= 0:
+                continue
+            m = np.uint64((i * 64 + (j + 32)) & 0xFFFF)
+            ctr = np.array([m * np.uint64(8) + np.uint64(k) for k in range(4)], dtype=np.uint64)
+            r = _urand(seed, ctr)
+            a1, a2 = float(r[0]), float(r[1])
+            p1, p2 = _PI * float(r[2]), _PI * float(r[3])
+            modes.append((i, j, 1.0 / float(i * i + j * j), a1, a2, p1, p2,
+                          (_TWO_PI * float(i)) / float(X), (_TWO_PI * float(j)) / float(Y)))
+    cx = np.array([[_math.cos((_TWO_PI * float((i * x) % X)) / float(X)) for x in range(X)] for i in range(mx + 1)])
+    sx = np.array([[_math.sin((_TWO_PI * float((i * x) % X)) / float(X)) for x in range(X)] for i in range(mx + 1)])
+    by = np.zeros((len(modes), 4, Y))
+    for m, md in enumerate(modes):
+        j = md[1]
+        for q, p in enumerate((md[5], md[6])):
+            for y in range(Y):
+                n = ((j * y) % Y + Y) % Y
+                arg = (_TWO_PI * float(n)) / float(Y) + p
+                by[m, 2 * q, y] = _math.cos(arg)
+                by[m, 2 * q + 1, y] = _math.sin(arg)
+    U = np.zeros(Z)
+    env = np.zeros(Z)
+    denv = np.zeros(Z)
+    for z in range(Z):
+        yy = float(z) + 0.5
+        yw = yy if yy < hh else float(Z) - yy
+        yplus = yw * uc / nu
+        uplus = yplus if yplus < 11.0 else (_math.log(yplus) * kinv + Bplus)
+        U[z] = uc * uplus
+        sn = _math.sin(_PI * (float(z) + 0.5) / float(Z))
+        cs = _math.cos(_PI * (float(z) + 0.5) / float(Z))
+        env[z] = sn * sn
+        denv[z] = (((2.0 * sn) * cs) * _PI) / float(Z)
+    S = 0.0
+    for z in range(Z):
+        d2, e2 = float(denv[z]) * float(denv[z]), float(env[z]) * float(env[z])
+        for md in modes:
+            _, _, sc, a1, a2, p1, p2, kx, ky = md
+            q1, q2 = a1 * ky, a2 * kx
+            term = (sc * sc) * (d2 * ((a1 * a1 + a2 * a2) * 0.5) +
+                                e2 * (((q1 * q1 + q2 * q2) - 2.0 * (q1 * q2) * _math.cos(p1 - p2)) * 0.5))
+            S = S + term
+    ucl = uc * (_math.log(hh * uc / nu) * kinv + Bplus)
+    nrm = amp * ucl / _math.sqrt(S / float(Z)) if S > 0.0 else 0.0
+    return dict(modes=modes, cx=cx, sx=sx, by=by, U=U, env=env, denv=denv, nrm=nrm, ucl=ucl, nu=nu)
+
+
+def turbulent_velocity(X, Y, Z, omega, uc, amp, seed, nmodes, xs=None, tables=None):
+    """Velocity (u0, u1, u2), each [len(xs)][Y][Z], of the generator at the
+    x-planes xs (default all), before the populations are formed."""
+    t = tables or turbulent_tables(X, Y, Z, omega, uc, amp, seed, nmodes)
+    xs = np.arange(X) if xs is None else np.asarray(xs)
+    shp = (len(xs), Y, Z)
+    env = np.broadcast_to(t["env"][None, None, :], shp)
+    denv = np.broadcast_to(t["denv"][None, None, :], shp)
+    ux = np.zeros(shp)
+    uy = np.zeros(shp)
+    uz = np.zeros(shp)
+    for m, (i, j, sc, a1, a2, p1, p2, kx, ky) in enumerate(t["modes"]):
+        cA = t["cx"][i][xs][:, None, None]
+        sA = t["sx"][i][xs][:, None, None]
+        cb1, sb1, cb2, sb2 = (t["by"][m, q][None, :, None] for q in range(4))
+        c1 = cA * cb1 - sA * sb1
+        s1 = sA * cb1 + cA * sb1
+        c2 = cA * cb2 - sA * sb2
+        s2 = sA * cb2 + cA * sb2
+        ux = ux + sc * (-((denv * a2) * c2))
+        uy = uy + sc * ((denv * a1) * c1)
+        uz = uz + sc * (env * ((a1 * ky) * s1 - (a2 * kx) * s2))
+    u0 = t["U"][None, None, :] + t["nrm"] * ux
+    return u0, t["nrm"] * uy, t["nrm"] * uz
+
+
+def device_turbulent(X, Y, Z, omega, uc, amp, seed, nmodes, xs=None):
+    """AoS [len(xs)][Y][Z+2][19] of the library's TURBULENT initial state
+    (FP64) at the x-planes xs (default all)."""
+    u0, u1, u2 = turbulent_velocity(X, Y, Z, omega, uc, amp, seed, nmodes, xs)
+    f = np.zeros(u0.shape[:2] + (Z + 2, 19))
+    f[:, :, 1:Z + 1, :] = feq_ref_order(u0, u1, u2)
+    return f
