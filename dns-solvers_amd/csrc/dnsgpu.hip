@@ -34,6 +34,39 @@ int set_err(int code, const char* fmt, ...)
 
 extern "C" const char* dnsgpu_last_error(void) { return g_err; }
 
+// Staging of dnsgpu_fields_noalloc: one static array per GPU the library runs
+// on (each device holds its own copy, at its own address), held until the
+// process ends.  In host code `scratch` names a host-side stand-in, never
+// the device memory: its device address is resolved once per device with
+// hipGetSymbolAddress (scratch_address, called from dnsgpu_create with the
+// handle's device current) and only that address is passed to kernels and
+// copies.  Every handle on a device shares the array; calls are serialised
+// per device by g_scratch_mutex.
+constexpr size_t kScratchDoubles = DNSGPU_FIELDS_NOALLOC_MAX_DOUBLES;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+
+namespace {
+constexpr int kMaxDevices = 64;
+void* g_scratch_ptr[kMaxDevices];
+std::mutex g_scratch_table_mutex;
+std::mutex g_scratch_mutex[kMaxDevices];
+}  // namespace
+
+// Device address of `scratch` on device dev (the current device), resolved
+// on the first call for that device.
+static int scratch_address(int dev, void** out)
+{
+    if (dev < 0 || dev >= kMaxDevices) return set_err(DNSGPU_EARG, "device %d beyond the scratch table", dev);
+    std::lock_guard<std::mutex> lock(g_scratch_table_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        HIPCHK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = p;
+    }
+    *out = g_scratch_ptr[dev];
+    return 0;
+}
+
 
 template <typename K>
 static int occupancy_blocks(K kernel, int block, int* out)
@@ -341,6 +374,9 @@ extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
     int rc = plan_handle(c, h);
     if (rc) return fail(rc);
     const Geom& g = h->g;
+    if (hipGetDevice(&h->device) != hipSuccess) return fail(set_err(DNSGPU_EHIP, "no current device"));
+    rc = scratch_address(h->device, &h->scratch);
+    if (rc) return fail(rc);
     if (hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking) != hipSuccess)
         return fail(set_err(DNSGPU_EHIP, "stream creation failed"));
     if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
@@ -1101,6 +1137,45 @@ extern "C" int dnsgpu_fields(dnsgpu* h, int x0, int nx, double* vel, double* den
     A.nx = nx;
     A.vel = h->stage;
     A.den = h->stage + nn * 3;
+    int grid = (int)std::min<size_t>((nn + 255) / 256, 65536);
+    with_storage(h->c.precision, [&](auto t) {
+        hipLaunchKernelGGL(fields_kernel<decltype(t)>, dim3(grid), dim3(256), 0, h->stream, A);
+    });
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(vel, A.vel, nn * 3 * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemcpyAsync(den, A.den, nn * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
+// dnsgpu_fields without any allocation: the staging is the device's static
+// scratch array.  The fields kernel writes every element of the nn * 4
+// doubles it stages (zeros at wall rows and solid nodes), so nothing is
+// cleared first.
+extern "C" int dnsgpu_fields_noalloc(dnsgpu* h, int x0, int nx, double* vel, double* den)
+{
+    if (!h || !vel || !den) return set_err(DNSGPU_EARG, "null argument");
+    int rc = check_slab(h, x0, nx);
+    if (rc) return rc;
+    const size_t nn = (size_t)nx * h->g.Y * (h->g.Z + 2);
+    if (nn * 4 > kScratchDoubles)
+        return set_err(DNSGPU_EARG,
+                       "dnsgpu_fields_noalloc: %d x-planes need %zu doubles of staging, more than the %zu of the "
+                       "static array (at most %zu x-planes of this grid per call)",
+                       nx, nn * 4, kScratchDoubles, kScratchDoubles / ((size_t)h->g.Y * (h->g.Z + 2) * 4));
+    if (!h->scratch) return set_err(DNSGPU_EHIP, "dnsgpu_fields_noalloc: scratch address not resolved");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex[h->device]);
+    FieldArgs A;
+    A.g = h->g;
+    A.cur = h->buf[h->cur];
+    A.mbot = h->mbot;
+    A.mtop = h->mtop;
+    A.bands = h->has_bands ? h->bands : nullptr;
+    A.local = h->poststream;
+    A.x0 = x0;
+    A.nx = nx;
+    A.vel = (double*)h->scratch;
+    A.den = A.vel + nn * 3;
     int grid = (int)std::min<size_t>((nn + 255) / 256, 65536);
     with_storage(h->c.precision, [&](auto t) {
         hipLaunchKernelGGL(fields_kernel<decltype(t)>, dim3(grid), dim3(256), 0, h->stream, A);

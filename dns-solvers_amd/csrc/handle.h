@@ -67,6 +67,8 @@ struct dnsgpu {
     size_t dev_bytes;
     hipEvent_t ev0, ev1;
     struct dnsens* owner;  // ensemble the handle is a member of, or null
+    int device;            // the device current at dnsgpu_create
+    void* scratch;         // device address of that device's static staging array
 };
 
 // Entry points that change or advance a member's state are refused while

@@ -27,7 +27,7 @@ EXPORTS = (
     "dnsgpu_commit_load", "dnsgpu_save_aos", "dnsgpu_get_force", "dnsgpu_step",
     "dnsgpu_statistics", "dnsgpu_fields", "dnsgpu_time_steps", "dnsgpu_device_bytes",
     "dnsgpu_autotune", "dnsgpu_tavg_reset", "dnsgpu_tavg_sample", "dnsgpu_tavg_get",
-    "dnsgpu_footprint",
+    "dnsgpu_footprint", "dnsgpu_fields_noalloc",
     "dnsens_create", "dnsens_destroy", "dnsens_member_bytes", "dnsens_footprint", "dnsens_add", "dnsens_remove",
     "dnsens_count", "dnsens_bytes_used", "dnsens_budget", "dnsens_step", "dnsens_info", "dnsens_rows",
     "dnsens_autotune", "dnsens_time_steps", "dnsens_time_kernels", "dnsens_grid",
@@ -120,6 +120,7 @@ def load():
         L.dnsgpu_step.argtypes = [vp, ctypes.c_long, ctypes.c_int, ctypes.POINTER(DnsStepLog)]
         L.dnsgpu_statistics.argtypes = [vp, dp]
         L.dnsgpu_fields.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp]
+        L.dnsgpu_fields_noalloc.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp]
         L.dnsgpu_time_steps.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp]
         L.dnsgpu_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), dp]
         L.dnsgpu_tavg_reset.argtypes = [vp]
@@ -289,6 +290,15 @@ class Channel:
         vel = np.zeros((self.X, self.Y, self.Z + 2, 3))
         den = np.zeros((self.X, self.Y, self.Z + 2))
         _chk(load().dnsgpu_fields(self.h, 0, self.X, _dp(vel), _dp(den)))
+        return vel, den
+
+    def fields_slab(self, x0, nx, noalloc=False):
+        """vel [nx][Y][Z+2][3], den [nx][Y][Z+2] of the x-planes [x0, x0 + nx)
+        through dnsgpu_fields, or dnsgpu_fields_noalloc (static staging)."""
+        vel = np.zeros((nx, self.Y, self.Z + 2, 3))
+        den = np.zeros((nx, self.Y, self.Z + 2))
+        fn = load().dnsgpu_fields_noalloc if noalloc else load().dnsgpu_fields
+        _chk(fn(self.h, x0, nx, _dp(vel), _dp(den)))
         return vel, den
 
     def time_steps(self, n, stats=False):

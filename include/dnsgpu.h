@@ -176,6 +176,17 @@ int dnsgpu_statistics(dnsgpu *h, double *out);
  * vel[nx][ydv][zdv+2][3], den[nx][ydv][zdv+2] (zeros on wall rows). */
 int dnsgpu_fields(dnsgpu *h, int x0, int nx, double *vel, double *den);
 
+/* dnsgpu_fields without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid it).  No counterpart in the reference.  The staging lives in a
+ * static array of DNSGPU_FIELDS_NOALLOC_MAX_DOUBLES doubles (64 MiB) per GPU,
+ * held until the process ends and shared by every handle on that GPU: a call
+ * takes nx x-planes with nx * ydv * (zdv+2) * 4 <= that number (DNSGPU_EARG
+ * beyond it, nothing changed).  Calls on one GPU are serialised inside the
+ * library; the array keeps the last call's contents. */
+#define DNSGPU_FIELDS_NOALLOC_MAX_DOUBLES ((size_t)8 << 20)
+int dnsgpu_fields_noalloc(dnsgpu *h, int x0, int nx, double *vel, double *den);
+
 /* Time averages, FP64 accumulators on the device.  Every logged step
  * (dnsgpu_step with rows) adds its row; dnsgpu_tavg_sample adds one sample
  * of the statistics.c plane profiles of the current state and of the
