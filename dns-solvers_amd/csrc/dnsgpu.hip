@@ -387,6 +387,11 @@ extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
         h->dev_bytes += a.n;
     }
     const size_t xy = (size_t)g.X * g.Y;
+    h->nf_host = (double*)malloc(sizeof(double) * g.Z);
+    if (!h->nf_host) return fail(set_err(DNSGPU_ENOMEM, "host allocation failed"));
+    for (int z = 0; z < g.Z; z++) h->nf_host[z] = (double)xy;
+    if (hipMemcpyAsync(h->nf, h->nf_host, sizeof(double) * g.Z, hipMemcpyHostToDevice, h->stream) != hipSuccess)
+        return fail(set_err(DNSGPU_EHIP, "initial copy failed"));
     h->ring_cap = kRingRows;
     if (hipMemsetAsync(h->mbot, -1, xy, h->stream) != hipSuccess ||
         hipMemsetAsync(h->mtop, -1, xy, h->stream) != hipSuccess ||
@@ -426,6 +431,7 @@ extern "C" void dnsgpu_destroy(dnsgpu* h)
     if (h->ev0) (void)hipEventDestroy(h->ev0);
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
+    free(h->nf_host);
     free(h);
 }
 
@@ -626,6 +632,20 @@ extern "C" int dnsgpu_set_solid_bands(dnsgpu* h, const int16_t* bottom, const in
     HIPCHK(hipStreamSynchronize(h->stream));
     h->has_bands = any ? 1 : 0;
     h->nfluid = total;
+    for (int z = 0; z < Z; z++) h->nf_host[z] = nf[z];
+    if (any) {
+        // the wall-adjacent velocity is sampled over the columns whose first
+        // row next to each wall is fluid (the walls are no-slip here)
+        int nb = 0, nt = 0;
+        for (size_t i = 0; i < xy; i++) {
+            nb += (bottom ? bottom[i] : 0) == 0;
+            nt += (top ? top[i] : 0) == 0;
+        }
+        h->slip_cols[0] = 0;
+        h->slip_cols[1] = nb;
+        h->slip_cols[2] = 0;
+        h->slip_cols[3] = nt;
+    }
     // the force of the first step and the plane-mean shifts were derived by
     // the last init or load over every node; derive them again over the
     // fluid nodes (solid nodes may hold anything, zeros in a checkpoint)
@@ -957,22 +977,30 @@ static int launch_stats_pass(dnsgpu* h, int pass)
     return 0;
 }
 
+// Fluid nodes of plane z as the divisor of its statistics.c means (1 for a
+// plane with none, whose sums are zero).
+static double plane_count(const dnsgpu* h, int z)
+{
+    const double n = h->nf_host[z];
+    return n > 0.0 ? n : 1.0;
+}
+
 // statistics.c's two passes over the current state: leaves the plane means
 // (UAVG, VAVG, WAVG, DAVG) in h->means and m, and the fluctuation sums of
 // pass 2 (u'u', v'v', w'w', u'w', rho'rho') in h->planes and s.
 static int run_statistics(dnsgpu* h, std::vector<double>& m, std::vector<double>& s)
 {
     const int Z = h->g.Z;
-    const double nxy = (double)(h->g.X * h->g.Y);
     s.assign(5 * (size_t)Z, 0.0);
     m.assign(4 * (size_t)Z, 0.0);
     int rc = with_storage(h->c.precision, [&](auto t) { return launch_stats_pass<decltype(t)>(h, 1); });
     if (rc) return rc;
     HIPCHK(hipMemcpyAsync(s.data(), h->planes, sizeof(double) * 5 * Z, hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
-    // statistics.c:54-60
+    // statistics.c:54-60, over the fluid nodes of each plane (the whole
+    // plane with flat walls)
     for (int q = 0; q < 4; q++)
-        for (int z = 0; z < Z; z++) m[(size_t)q * Z + z] = s[(size_t)q * Z + z] / nxy;
+        for (int z = 0; z < Z; z++) m[(size_t)q * Z + z] = s[(size_t)q * Z + z] / plane_count(h, z);
     HIPCHK(hipMemcpyAsync(h->means, m.data(), sizeof(double) * 4 * Z, hipMemcpyHostToDevice, h->stream));
     rc = with_storage(h->c.precision, [&](auto t) { return launch_stats_pass<decltype(t)>(h, 2); });
     if (rc) return rc;
@@ -985,13 +1013,13 @@ extern "C" int dnsgpu_statistics(dnsgpu* h, double* out)
 {
     if (!h || !out) return set_err(DNSGPU_EARG, "null argument");
     const int Z = h->g.Z, zd = Z + 2;
-    const double nxy = (double)(h->g.X * h->g.Y);
     std::vector<double> s, m;
     int rc = run_statistics(h, m, s);
     if (rc) return rc;
     memset(out, 0, sizeof(double) * 9 * zd);
     for (int z = 0; z < Z; z++) {
         const int k = z + 1;
+        const double nxy = plane_count(h, z);
         out[0 * zd + k] = m[0 * (size_t)Z + z];
         out[1 * zd + k] = m[1 * (size_t)Z + z];
         out[2 * zd + k] = m[2 * (size_t)Z + z];
@@ -1052,7 +1080,7 @@ extern "C" int dnsgpu_tavg_sample(dnsgpu* h)
     if (rc) return rc;
     const int Z = h->g.Z;
     hipLaunchKernelGGL(tavg_profile_kernel, dim3((Z + 255) / 256), dim3(256), 0, h->stream, h->means, h->planes, Z,
-                       (double)h->g.X * h->g.Y, h->tprof, h->st);
+                       h->nf, h->tprof, h->st);
     HIPCHK(hipGetLastError());
     rc = with_storage(h->c.precision, [&](auto t) { return launch_slip<decltype(t)>(h); });
     if (rc) return rc;

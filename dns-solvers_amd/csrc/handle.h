@@ -38,7 +38,8 @@ struct dnsgpu {
     uint32_t* bands;  // [X*Y] solid band words hb | ht << 16
     uint32_t* bnbr;   // [X*Y][16] band neighbourhood records
     uint32_t* cnbr;   // [X*Y][16] column neighbourhood records
-    double* nf;       // [Z] fluid nodes per plane
+    double* nf;       // [Z] fluid nodes per plane (device)
+    double* nf_host;  // [Z] the same on the host
     double nfluid;    // fluid nodes in total
     int has_bands;
     double* part;     // [kNQ][Z][nblk]

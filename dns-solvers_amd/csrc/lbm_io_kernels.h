@@ -252,13 +252,16 @@ static __global__ __launch_bounds__(kWave* kMaxWaves) void stats_kernel(StatArgs
 //   tprof[4..6][z] += plane means of u^2, v^2, w^2  (variance + mean^2)
 //   tprof[7][z]    += plane mean of u w            (covariance + product)
 //   tprof[8][z]    += plane mean of rho^2
-// so the time-averaged rms is sqrt(<u^2> - <u>^2) over planes and time.
+// so the time-averaged rms is sqrt(<u^2> - <u>^2) over planes and time; the
+// plane means are over the nf[z] fluid nodes of each plane.
 static __global__ __launch_bounds__(256) void tavg_profile_kernel(const double* __restrict__ means,
-                                                           const double* __restrict__ fl, int Z, double nxy,
+                                                           const double* __restrict__ fl, int Z,
+                                                           const double* __restrict__ nf,
                                                            double* __restrict__ tprof, DevState* st)
 {
     const int z = blockIdx.x * blockDim.x + threadIdx.x;
     if (z < Z) {
+        const double nxy = nf[z] > 0.0 ? nf[z] : 1.0;
         const double u = means[z], v = means[Z + z], w = means[2 * Z + z], r = means[3 * Z + z];
         tprof[0 * Z + z] += u;
         tprof[1 * Z + z] += v;

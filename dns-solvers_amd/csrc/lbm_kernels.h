@@ -905,16 +905,21 @@ __device__ __forceinline__ void finalize_body(const FinalizeArgs& A, FinalizeSha
     const bool have_planes = A.planes != nullptr;
     double uz = 0.0, kz = 0.0;
     if (have_planes && t < Z) {
-        // plane means over the whole plane with solid nodes counting as
-        // u = 0 (encalc.c:54-59 divides by the full plane); sums run over the
-        // nf fluid nodes, shifted by c:  mean = (nf c + S1) / Nxy and
-        // sum_f |u - mean|^2 = S2 - 2 d.S1 + nf |d|^2 with d = mean - c
+        // The sums run over the nf fluid nodes of plane t, shifted by c.
+        // The flow rate and the wall stress take the plane means over the
+        // whole plane, solid nodes counting as u = 0 (encalc.c:54-59 and
+        // wshearstr.c divide by the full plane): uav = (nf c + S1) / Nxy.
+        // K takes the fluctuations about the mean over the fluid nodes,
+        // m = (nf c + S1) / nf, which is also the next shift:
+        //   sum_f |u - m|^2 = S2 - 2 d.S1 + nf |d|^2 with d = m - c.
+        // With flat walls nf = Nxy and both are the reference's.
         const double nf = A.nf ? A.nf[t] : Nxy;
         const double s1u = A.planes[0 * Z + t], s1v = A.planes[1 * Z + t], s1w = A.planes[2 * Z + t];
         const double cu = A.shift[t], cv = A.shift[Z + t], cw = A.shift[2 * Z + t];
-        const double mu_ = (nf * cu + s1u) / Nxy, mv = (nf * cv + s1v) / Nxy, mw = (nf * cw + s1w) / Nxy;
+        const double nfi = nf > 0.0 ? nf : 1.0;
+        const double mu_ = (nf * cu + s1u) / nfi, mv = (nf * cv + s1v) / nfi, mw = (nf * cw + s1w) / nfi;
         const double du = mu_ - cu, dv = mv - cv, dw = mw - cw;
-        uz = mu_;
+        uz = (nf * cu + s1u) / Nxy;
         kz = 2.0 * ((du * s1u + dv * s1v) + dw * s1w) - nf * ((du * du + dv * dv) + dw * dw);
         uav[t] = uz;
         dav[t] = (nf + A.planes[3 * Z + t]) / Nxy;
@@ -940,7 +945,7 @@ __device__ __forceinline__ void finalize_body(const FinalizeArgs& A, FinalizeSha
             for (int z = 0; z < Z; z++) srho += A.planes[3 * Z + z];
             const double denst = (A.nfluid + srho) / N;
             const double momentum = jx / N;
-            const double K = 0.5 * (A.globals[3] - sK[0]) / N;
+            const double K = 0.5 * (A.globals[3] - sK[0]) / A.nfluid;  // per fluid node
             const double U = sU[0] / (double)Z;
             const double stdown = (-2. * uav[0] + 3. * uav[1] - uav[2]) * (A.mu * dav[0]);
             const double stup = -(2. * uav[Z - 1] - 3. * uav[Z - 2] + uav[Z - 3]) * (A.mu * dav[Z - 3]);

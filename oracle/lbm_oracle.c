@@ -59,6 +59,11 @@ void oracle_encalc(const oracle_case *c, const double *ft, long cntr, oracle_log
     double *uavg = calloc((size_t)zd, sizeof(double));
     double *vavg = calloc((size_t)zd, sizeof(double));
     double *wavg = calloc((size_t)zd, sizeof(double));
+    double *ui = calloc((size_t)zd, sizeof(double));
+    double *vi = calloc((size_t)zd, sizeof(double));
+    double *wi = calloc((size_t)zd, sizeof(double));
+    long *nf = calloc((size_t)zd, sizeof(long));
+    long nfluid = 0;
     double momentum = 0., denst = 0., K = 0., U = 0.;
     const double dt = log_dt(c);
 
@@ -83,8 +88,18 @@ void oracle_encalc(const oracle_case *c, const double *ft, long cntr, oracle_log
                 uavg[k] += ux;
                 vavg[k] += uy;
                 wavg[k] += uz;
+                nf[k]++;
+                nfluid++;
             }
-    for (int k = 0; k < zd; k++) { /* encalc.c:54-59 */
+    /* encalc.c:54-59 divides the plane sums by the whole plane: the
+     * superficial means (solid nodes count as u = 0), kept for the flow
+     * rate; the fluctuations of K are taken about the means over the fluid
+     * nodes of the plane and K per fluid node (with flat walls every node of
+     * a plane is fluid and both are the reference's). */
+    for (int k = 0; k < zd; k++) {
+        ui[k] = nf[k] ? uavg[k] / ((double)nf[k]) : 0.;
+        vi[k] = nf[k] ? vavg[k] / ((double)nf[k]) : 0.;
+        wi[k] = nf[k] ? wavg[k] / ((double)nf[k]) : 0.;
         uavg[k] /= ((double)(X * Y));
         vavg[k] /= ((double)(X * Y));
         wavg[k] /= ((double)(X * Y));
@@ -108,10 +123,10 @@ void oracle_encalc(const oracle_case *c, const double *ft, long cntr, oracle_log
                 ux /= dens;
                 uy /= dens;
                 uz /= dens;
-                double uf = ux - uavg[k], vf = uy - vavg[k], wf = uz - wavg[k];
+                double uf = ux - ui[k], vf = uy - vi[k], wf = uz - wi[k];
                 K += 0.5 * (uf * uf + vf * vf + wf * wf);
             }
-    K /= ((double)(X * Y * (zd - 2)));
+    K /= ((double)nfluid);
     for (int k = 0; k < zd; k++) U += uavg[k]; /* encalc.c:116-119 */
     U /= ((double)(zd - 2));
 
@@ -125,6 +140,10 @@ void oracle_encalc(const oracle_case *c, const double *ft, long cntr, oracle_log
     free(uavg);
     free(vavg);
     free(wavg);
+    free(ui);
+    free(vi);
+    free(wi);
+    free(nf);
 }
 
 int oracle_wshearstr(const oracle_case *c, const double *ft, long cntr, oracle_log *row)
@@ -309,6 +328,10 @@ void oracle_statistics(const oracle_case *c, const double *ft, double *out)
     double *UAVG = out + 0 * zd, *VAVG = out + 1 * zd, *WAVG = out + 2 * zd;
     double *UTIN = out + 3 * zd, *VTIN = out + 4 * zd, *WTIN = out + 5 * zd;
     double *REYS = out + 6 * zd, *DAVG = out + 7 * zd, *DTIN = out + 8 * zd;
+    /* statistics.c:31-147 over the nodes of each plane, which with solid
+     * bands inside the rows are its fluid nodes (the means and the rms over
+     * the fluid; with flat walls the whole plane, as the reference) */
+    long *nf = calloc((size_t)zd, sizeof(long));
     memset(out, 0, sizeof(double) * 9u * (size_t)zd);
     for (int i = 0; i < X; i++) /* statistics.c:31-52 */
         for (int j = 0; j < Y; j++)
@@ -326,12 +349,14 @@ void oracle_statistics(const oracle_case *c, const double *ft, double *out)
                 VAVG[k] += (v / dens);
                 WAVG[k] += (w / dens);
                 DAVG[k] += dens;
+                nf[k]++;
             }
     for (int k = 0; k < zd; k++) {
-        UAVG[k] /= (double)(X * Y);
-        VAVG[k] /= (double)(X * Y);
-        WAVG[k] /= (double)(X * Y);
-        DAVG[k] /= (double)(X * Y);
+        const double n = nf[k] ? (double)nf[k] : 1.;
+        UAVG[k] /= n;
+        VAVG[k] /= n;
+        WAVG[k] /= n;
+        DAVG[k] /= n;
     }
     for (int i = 0; i < X; i++) /* statistics.c:74-100 */
         for (int j = 0; j < Y; j++)
@@ -356,12 +381,14 @@ void oracle_statistics(const oracle_case *c, const double *ft, double *out)
                 DTIN[k] += DF * DF;
             }
     for (int k = 0; k < zd; k++) {
-        UTIN[k] /= (double)(X * Y);
-        VTIN[k] /= (double)(Y * X);
-        WTIN[k] /= (double)(Y * X);
-        REYS[k] /= (double)(X * Y);
-        DTIN[k] /= (double)(X * Y);
+        const double n = nf[k] ? (double)nf[k] : 1.;
+        UTIN[k] /= n;
+        VTIN[k] /= n;
+        WTIN[k] /= n;
+        REYS[k] /= n;
+        DTIN[k] /= n;
     }
+    free(nf);
     for (int k = 0; k < zd; k++) {
         UTIN[k] = sqrt(UTIN[k]);
         VTIN[k] = sqrt(VTIN[k]);

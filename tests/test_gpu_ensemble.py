@@ -19,6 +19,7 @@ import pytest
 
 import dnsgpu
 import fields
+import pyoracle
 
 pytestmark = pytest.mark.gpu
 
@@ -37,37 +38,69 @@ def make(shape, texture, forcing=dnsgpu.CONST_Q, F=0.0, precision=dnsgpu.FP64, s
     elif texture == "riblets":
         hb = fields.riblet_bands(X, Y, 8, 4)
         ch.set_solid_bands(hb, hb)
+    elif texture == "blocks":
+        ch.set_solid_bands(fields.block_bands(X, Y, 4, 4, 2, 2, 3), fields.block_bands(X, Y, 6, 4, 3, 1, 2, 1))
     ch.init_field(dnsgpu.INIT_POISEUILLE, uc=0.06, amp=0.05, seed=seed)
     return ch
 
 
-CASES = [((16, 8, 10), "smooth"), ((8, 16, 24), "riblets"), ((16, 12, 70), "stripes"), ((8, 8, 130), "posts")]
+def oracle_of(ch, shape, texture, forcing, F):
+    """The oracle case of a member built by make()."""
+    X, Y, Z = shape
+    if texture == "posts":
+        m = fields.posts_mask(X, Y)
+        mask = pyoracle.wall_mask(X, Y, Z, m, m)
+    elif texture == "stripes":
+        mask = pyoracle.wall_mask(X, Y, Z, fields.stripes_mask(X, Y, 6, 3), fields.stripes_mask(X, Y, 4, 1, offset=2))
+    elif texture == "riblets":
+        hb = fields.riblet_bands(X, Y, 8, 4)
+        mask = fields.band_mask(X, Y, Z, hb, hb)
+    elif texture == "blocks":
+        mask = fields.band_mask(X, Y, Z, fields.block_bands(X, Y, 4, 4, 2, 2, 3),
+                                fields.block_bands(X, Y, 6, 4, 3, 1, 2, 1))
+    else:
+        mask = pyoracle.wall_mask(X, Y, Z)
+    o = pyoracle.Oracle(X, Y, Z, pyoracle.omega(REBS, UB, Z), UB, 0.25, TM=5,
+                        forcing=1 if forcing == dnsgpu.CONST_F else 0, F=F, mask=mask)
+    return o, mask
+
+
+CASES = [((16, 8, 10), "smooth"), ((8, 16, 24), "riblets"), ((16, 12, 70), "stripes"), ((8, 8, 130), "posts"),
+         ((12, 16, 20), "blocks")]
 
 
 def rel(a, b):
     return float(np.abs(a - b).max() / np.abs(b).max())
 
 
+@pytest.mark.parametrize("precision", [dnsgpu.FP64, dnsgpu.FP32, dnsgpu.FP16])
 @pytest.mark.parametrize("forcing", [dnsgpu.CONST_F, dnsgpu.CONST_Q])
-def test_members_match_single_case_path(forcing):
-    print(f"case: ensemble vs single, forcing {forcing}", flush=True)
+def test_members_match_single_case_path(forcing, precision):
+    """Every member, flat walls, slip textures and solid bands (riblets,
+    blocks), in each storage precision: the same state as the single-case
+    path of the same precision (bitwise under constant force; under constant
+    flow rate the force is a global sum taken in another order: 1e-12 in
+    FP64, 1e-6 absolute in FP32 and FP16 storage)."""
+    print(f"case: ensemble vs single, forcing {forcing}, precision {precision}", flush=True)
     n = 40
     F = 2e-5 if forcing == dnsgpu.CONST_F else 0.0
-    ens = dnsgpu.Ensemble(precision=dnsgpu.FP64)
-    members = [make(s, t, forcing, F) for s, t in CASES]
+    ens = dnsgpu.Ensemble(precision=precision)
+    members = [make(s, t, forcing, F, precision=precision) for s, t in CASES]
     for ch in members:
         ens.add(ch)
     ens.step(n)
     for (shape, tex), ch in zip(CASES, members):
-        ref = make(shape, tex, forcing, F)
+        ref = make(shape, tex, forcing, F, precision=precision)
         rl = ref.step(n + 1, cntr0=0)  # dnsens_add takes the first step
         el = ens.rows(ch, n)
         a = ch.save_aos()[0]
         b = ref.save_aos()[0]
         if forcing == dnsgpu.CONST_F:
             assert np.array_equal(a, b), (shape, tex, rel(a, b))
-        else:
+        elif precision == dnsgpu.FP64:
             assert rel(a, b) <= 1e-12, (shape, tex, rel(a, b))
+        else:
+            assert np.abs(a - b).max() <= 1e-6, (shape, tex, rel(a, b))
         for k in ("K", "flowrate", "massrate", "stup", "stdown", "stress", "pgrad", "force", "rho"):
             ref_k = rl[k][1:]
             assert np.abs(el[k] - ref_k).max() <= 1e-10 * np.abs(ref_k).max() + 1e-300, (shape, tex, k)
@@ -80,6 +113,41 @@ def test_members_match_single_case_path(forcing):
         ref.close()
     info = ens.info(members[0])
     assert info["finite"] == 1 and info["steps"] == n + 1
+    ens.close()
+
+
+@pytest.mark.parametrize("forcing", [dnsgpu.CONST_F, dnsgpu.CONST_Q])
+def test_members_match_the_oracle(forcing):
+    """Each FP64 member's state after its steps in the mix against the
+    oracle's run of the reference loop (DNS-solver/src/dns-const-q-xycut.c:
+    487-518) from the same seeded field: bitwise under constant force, within
+    1e-12 relative under constant flow rate; log rows within 1e-10."""
+    print(f"case: ensemble members vs oracle, forcing {forcing}", flush=True)
+    n = 30
+    F = 2e-5 if forcing == dnsgpu.CONST_F else 0.0
+    ens = dnsgpu.Ensemble(precision=dnsgpu.FP64)
+    members = [make(s, t, forcing, F) for s, t in CASES]
+    for ch in members:
+        ens.add(ch)
+    ens.step(n)
+    for (shape, tex), ch in zip(CASES, members):
+        X, Y, Z = shape
+        o, mask = oracle_of(ch, shape, tex, forcing, F)
+        f0 = fields.device_poiseuille(X, Y, Z, 0.06, 0.05, 7)
+        f0[mask != 0] = 0.0
+        oft, of = f0.copy(), f0.copy()
+        ol = o.run(oft, of, n + 1)  # the add's first step and n more
+        ft = ch.save_aos()[0]
+        if forcing == dnsgpu.CONST_F:
+            assert np.array_equal(ft, oft), (shape, tex, rel(ft, oft))
+        else:
+            assert rel(ft, oft) <= 1e-12, (shape, tex, rel(ft, oft))
+        el = ens.rows(ch, n)
+        for k in ("K", "flowrate", "massrate", "stup", "stdown", "stress", "pgrad", "rho"):
+            ref_k = ol[k][1:]
+            assert np.abs(el[k] - ref_k).max() <= 1e-10 * np.abs(ref_k).max() + 1e-300, (shape, tex, k)
+    for ch in members:
+        ch.close()
     ens.close()
 
 
@@ -113,7 +181,7 @@ def test_alone_equals_in_mix(precision):
     solve run alone and in a changing mix (admissions and retirements at
     other steps) is bitwise identical."""
     print(f"case: alone vs mix, precision {precision}", flush=True)
-    probe_shape, probe_tex = (16, 12, 70), "stripes"
+    probe_shape, probe_tex = (12, 24, 40), "riblets"
     alone = dnsgpu.Ensemble(precision=precision)
     p1 = make(probe_shape, probe_tex, precision=precision, seed=11)
     alone.add(p1)

@@ -419,3 +419,37 @@ def test_momentum_conserved_with_slip_walls(precision):
     drift = float(np.abs(j - j[0]).max() / abs(j[0]))
     print(f"relative momentum drift {drift:.3e}", flush=True)
     assert drift <= MOMENTUM_BARS[precision]
+
+
+@pytest.mark.parametrize("precision", [dnsgpu.FP64, dnsgpu.FP32, dnsgpu.FP16])
+def test_statistics_over_the_fluid_of_banded_planes(precision):
+    """A velocity uniform over the fluid of a channel with riblets on one
+    wall and blocks on the other: no fluctuation in any plane (the rms,
+    <u'w'>, the density rms and K vanish), the means over the fluid are U and
+    1, and the FP64 statistics and log row equal the oracle's."""
+    import test_oracle_invariants as inv
+    X, Y, Z, U = 6, 16, 20, 0.04
+    print(f"case: uniform banded state, precision {precision}", flush=True)
+    hb = fields.riblet_bands(X, Y, 8, 4)
+    ht = fields.block_bands(X, Y, 3, 4, 1, 2, 3)
+    f, s = inv.uniform_banded_state(X, Y, Z, hb, ht, U)
+    with dnsgpu.Channel(X, Y, Z, Rebs=60.0, ub=0.05, CFL=0.25, forcing=dnsgpu.CONST_F, force=0.0,
+                        precision=precision) as ch:
+        ch.load_aos(f, f)
+        ch.set_solid_bands(hb, ht)
+        st = ch.statistics()
+        row = ch.step(1)
+    fl = slice(1, Z + 1)
+    assert np.abs(st[3:7, fl]).max() <= 1e-14 * U
+    assert np.abs(st[8, fl]).max() <= 1e-13
+    # the stored populations of U: exact in FP64, rounded to FP32 / FP16
+    tu, tr = {dnsgpu.FP64: (1e-12, 1e-14), dnsgpu.FP32: (1e-6, 1e-7), dnsgpu.FP16: (1e-3, 1e-4)}[precision]
+    assert np.allclose(st[0, fl], U, rtol=tu) and np.allclose(st[7, fl], 1.0, rtol=tr)
+    assert abs(row["K"][0]) <= 1e-28
+    if precision == dnsgpu.FP64:
+        o = pyoracle.Oracle(X, Y, Z, pyoracle.omega(60.0, 0.05, Z), 0.05, 0.25, forcing=1, F=0.0, mask=s)
+        assert np.allclose(st, o.statistics(f), rtol=1e-12, atol=1e-15)  # rho rms: rounding of sums of ones
+        ol = o.logs_of(f)
+        # the wall stresses of a uniform velocity are rounding (1e-19) on both sides
+        for k in ("flowrate", "massrate", "stup", "stdown", "rho"):
+            assert np.isclose(row[k][0], ol[k], rtol=1e-12, atol=1e-17), k

@@ -92,3 +92,27 @@ def test_dnsserve_command_line(tmp_path):
     for s in solves:
         assert s["status"] in (0, 1) and 201 <= s["steps"] <= 401 and s["u_tau"] > 0
     assert "admit" in r.stderr and "done" in r.stderr
+
+
+def test_texture_sweep_of_64_requests_alone_and_in_the_mix():
+    """The configured sweep: 64 seeded requests (grids {128,256,512} x
+    {64,128,256} x {64,96,128}, Re_tau in [90,180], grooves or riblets),
+    FP32-shifted storage, all resident together, with a short transient,
+    window and cap; a sample of the solves (the first groove, the first
+    riblet and the largest grid) served alone gives bitwise the same
+    results as in the mix of 64."""
+    print("case: serve the 64-request sweep, FP32, cap 400", flush=True)
+    reqs = dnsserve.generate(64, seed=5, mean_gap=2.0)
+    c = cfg(transient=200, window=200, check=100, cap=400, precision=dnsgpu.FP32, init_modes=8)
+    mix, summ = dnsserve.sweep(reqs, c)
+    assert summ["completed"] == 64 and summ["max_resident"] == 64, [r["status"] for r in mix]
+    for r in mix:
+        assert r["status"] in (0, 1) and 201 <= r["steps"] <= 401 and r["utau"] > 0, r
+    tex = [r.texture for r in reqs]
+    sizes = [r.xdv * r.ydv * r.zdv for r in reqs]
+    sample = sorted({tex.index(dnsserve.GROOVES), tex.index(dnsserve.RIBLETS), sizes.index(max(sizes))})
+    for i in sample:
+        one = dnsserve.requests([{k: getattr(reqs[i], k) for k, _ in dnsserve.Request._fields_}])
+        alone, _ = dnsserve.sweep(one, c)
+        for k in KEYS:
+            assert alone[0][k] == mix[i][k], (i, k)

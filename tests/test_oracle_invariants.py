@@ -237,3 +237,39 @@ def test_block_streaming_exactly_once_and_mass(blocks):
     o.run(ft, f, 30)
     assert abs(ft.sum() - m0) < 1e-13 * m0
     assert (ft[~fluid] == 0).all()
+
+
+def uniform_banded_state(X, Y, Z, hb, ht, U=0.04):
+    """feq(1, (U, 0, 0)) at every fluid node, zeros at wall rows and band nodes."""
+    s = fields.band_mask(X, Y, Z, hb, ht)
+    u = np.zeros((X, Y, Z + 2, 3))
+    u[..., 0] = U
+    f = fields.feq(np.ones((X, Y, Z + 2)), u)
+    f[s != 0] = 0.0
+    return np.ascontiguousarray(f), s
+
+
+def test_statistics_over_the_fluid_of_banded_planes():
+    # a velocity uniform over the fluid has no fluctuation in any plane, band
+    # planes included: the rms, <u'w'>, the density rms and K vanish (a plane
+    # mean over the whole plane with solids as zero would give an rms of
+    # sqrt(phi) (1 - phi) U at fluid fraction phi), the means over the fluid
+    # are U and 1, and the flow rate counts the fluid only
+    X, Y, Z, U = 6, 16, 20, 0.04
+    hb = fields.riblet_bands(X, Y, 8, 4)
+    ht = fields.block_bands(X, Y, 3, 4, 1, 2, 3)
+    f, s = uniform_banded_state(X, Y, Z, hb, ht, U)
+    o = pyoracle.Oracle(X, Y, Z, 1.2, 0.05, 0.25, forcing=1, F=0.0, mask=s)
+    st = o.statistics(f)
+    fl = slice(1, Z + 1)
+    assert np.abs(st[3:7, fl]).max() <= 1e-15 * U
+    assert np.abs(st[8, fl]).max() <= 1e-14
+    assert np.allclose(st[0, fl], U, rtol=1e-14) and np.allclose(st[7, fl], 1.0, rtol=1e-14)
+    row = o.logs_of(f)
+    assert abs(row["K"]) <= 1e-30
+    fluid = (s[:, :, fl] == 0).sum()
+    assert abs(row["flowrate"] - U * fluid / X) <= 1e-13 * U * fluid / X
+    # the same state with flat walls: the reference's plane statistics
+    f2, s2 = uniform_banded_state(X, Y, Z, np.zeros_like(hb), np.zeros_like(ht), U)
+    o2 = pyoracle.Oracle(X, Y, Z, 1.2, 0.05, 0.25, forcing=1, F=0.0, mask=s2)
+    assert abs(o2.logs_of(f2)["flowrate"] - U * Y * Z) <= 1e-13 * U * Y * Z
