@@ -82,6 +82,7 @@ def run_precision(wname, prec, steps, warmup):
         else:
             ch.init_field(dnsgpu.INIT_TURBULENT, uc=w["utau"], amp=0.1, seed=2, nmodes=8)
         blocks, _ = ch.autotune(trial_steps=5)
+        _, waves = ch.step_shape()
         ch.step(warmup, cntr0=0, logs=True)
         t0 = time.perf_counter()
         logs = ch.step(steps, cntr0=warmup, logs=True)  # returns after the stream drained
@@ -109,6 +110,7 @@ def run_precision(wname, prec, steps, warmup):
         "nodes": nodes,
         "finite": ok,
         "step_blocks": blocks,
+        "step_waves_per_block": waves,
         "stress_last": float(logs["stress"][-1]),
     }
 

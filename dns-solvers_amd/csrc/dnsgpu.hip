@@ -258,6 +258,8 @@ static int plan_handle(const dns_case* c, dnsgpu* h)
     g.total = (long long)g.ncols * Q * g.Z;
     h->block = kWave * g.nzc * g.cpb;
     h->step_w = step_waves(g.Z);
+    const char* swe = getenv("DNSGPU_STEP_WAVES");
+    if (swe) h->step_w = std::max(1, std::min(atoi(swe), h->step_w));
     h->esize = storage_bytes(c->precision);
     // omega and the constants exactly as dns-const-q-xycut.c:211-217,
     // wshearstr.c:24, encalc.c:20 evaluate them.
@@ -284,8 +286,8 @@ static int plan_handle(const dns_case* c, dnsgpu* h)
 
     // grids: the chunk-mapped kernels fill the device once; the step kernel
     // keeps about 4 (FP64) or 8 (FP32) waves per CU in flight, the measured
-    // optimum of the read+write stream (overridden by the autotuner or
-    // DNSGPU_STEP_BLOCKS)
+    // optimum of the read+write stream (overridden by the autotuner, or by
+    // DNSGPU_STEP_BLOCKS and DNSGPU_STEP_WAVES)
     int occ = 0;
     int rc = with_storage(c->precision, [&](auto t) {
         return occupancy_blocks(moments_kernel<decltype(t), false>, h->block, &occ);
@@ -310,8 +312,9 @@ static int plan_handle(const dns_case* c, dnsgpu* h)
     sb = std::min(sb, h->step_resident);
     const char* sbe = getenv("DNSGPU_STEP_BLOCKS");
     if (sbe) sb = atoi(sbe);
-    h->step_nblk_max = std::max(1, (g.ncols + h->step_w - 1) / h->step_w);
-    h->step_nblk_max = std::min(h->step_nblk_max, 64 * ncu);
+    // the partial-sum buffers hold one row per block for any block shape
+    // the autotuner may choose (down to one wave per block)
+    h->step_nblk_max = std::max(1, std::min(g.ncols, 64 * ncu));
     h->step_nblk = std::max(1, std::min(sb, h->step_nblk_max));
     return 0;
 }
@@ -449,15 +452,19 @@ struct TuneKey {
 };
 // shared by every handle of the process (handles may live on different host
 // threads): guarded
-std::vector<std::pair<TuneKey, int>> g_tune_cache;
+// a tuned launch shape: blocks of the grid and waves per block
+struct TuneShape {
+    int blocks, waves;
+};
+std::vector<std::pair<TuneKey, TuneShape>> g_tune_cache;
 std::mutex g_tune_mutex;
 
-bool tune_lookup(const TuneKey& k, int* blocks)
+bool tune_lookup(const TuneKey& k, TuneShape* out)
 {
     std::lock_guard<std::mutex> lock(g_tune_mutex);
     for (auto& e : g_tune_cache)
         if (e.first == k) {
-            *blocks = e.second;
+            *out = e.second;
             return true;
         }
     const char* path = getenv("DNSGPU_TUNE_CACHE");
@@ -465,85 +472,142 @@ bool tune_lookup(const TuneKey& k, int* blocks)
     FILE* f = fopen(path, "r");
     if (!f) return false;
     TuneKey r;
-    int b;
+    TuneShape sh;
     bool found = false;
-    while (fscanf(f, "%d %d %d %d %d %d", &r.X, &r.Y, &r.Z, &r.prec, &r.bands, &b) == 6)
+    while (fscanf(f, "%d %d %d %d %d %d %d", &r.X, &r.Y, &r.Z, &r.prec, &r.bands, &sh.blocks, &sh.waves) == 7)
         if (r == k) {
-            *blocks = b;
+            *out = sh;
             found = true;
         }
     fclose(f);
-    if (found) g_tune_cache.push_back({k, *blocks});
+    if (found) g_tune_cache.push_back({k, *out});
     return found;
 }
 
-void tune_store(const TuneKey& k, int blocks)
+void tune_store(const TuneKey& k, TuneShape sh)
 {
     std::lock_guard<std::mutex> lock(g_tune_mutex);
-    g_tune_cache.push_back({k, blocks});
+    g_tune_cache.push_back({k, sh});
     const char* path = getenv("DNSGPU_TUNE_CACHE");
     if (!path) return;
     FILE* f = fopen(path, "a");
     if (!f) return;
-    fprintf(f, "%d %d %d %d %d %d\n", k.X, k.Y, k.Z, k.prec, k.bands, blocks);
+    fprintf(f, "%d %d %d %d %d %d %d\n", k.X, k.Y, k.Z, k.prec, k.bands, sh.blocks, sh.waves);
     fclose(f);
 }
 }  // namespace
+
+// Sets the step kernel's launch shape (W waves per block, nblk blocks)
+// within what the LDS, the device and the partial buffers allow.
+static int set_step_shape(dnsgpu* h, int W, int nblk)
+{
+    W = std::max(1, std::min(W, step_waves(h->g.Z)));
+    int per = 0;
+    const hipError_t oe = with_storage(h->c.precision, [&](auto t) {
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, step_kernel<decltype(t), false, true, true>,
+                                                            kWave * W, step_lds_bytes(h->g.Z, W, true));
+    });
+    HIPCHK(oe);
+    h->step_w = W;
+    h->step_resident = std::max(1, per) * h->ncu;
+    const int cols = std::max(1, (h->g.ncols + W - 1) / W);
+    h->step_nblk = std::max(1, std::min(nblk, std::min(std::min(h->step_nblk_max, h->step_resident), cols)));
+    return 0;
+}
+
+// Time `trial` launches of the step kernel at the current shape (one
+// untimed launch first; the buffers are not flipped).
+static int time_shape(dnsgpu* h, int trial, double* ms_per_launch)
+{
+    for (int r = 0; r <= trial; r++) {
+        if (r == 1) HIPCHK(hipEventRecord(h->ev0, h->stream));
+        with_storage(h->c.precision, [&](auto t) { enqueue_step_kernel<decltype(t)>(h, true); });
+        HIPCHK(hipGetLastError());
+    }
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    HIPCHK(hipEventSynchronize(h->ev1));
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    *ms_per_launch = ms / trial;
+    return 0;
+}
 
 extern "C" int dnsgpu_autotune(dnsgpu* h, int trial_steps, int* best_blocks, double* best_ms)
 {
     if (!h || trial_steps < 1) return set_err(DNSGPU_EARG, "bad argument");
     NOT_MEMBER(h);
     const TuneKey key{h->g.X, h->g.Y, h->g.Z, h->c.precision, h->has_bands};
-    int cached = 0;
+    TuneShape cached;
     if (tune_lookup(key, &cached)) {
-        h->step_nblk = std::max(1, std::min(cached, std::min(h->step_nblk_max, h->step_resident)));
+        int rc = set_step_shape(h, cached.waves, cached.blocks);
+        if (rc) return rc;
         if (best_blocks) *best_blocks = h->step_nblk;
         if (best_ms) *best_ms = -1.0;
         return 0;
     }
-    // time the steady-state kernel; the handle's state flag and grid are
-    // restored on every return
+    // time the steady-state kernel over the shapes (waves per block x
+    // waves per CU), then the three fastest again, alternating, to keep a
+    // noisy first round from choosing; the handle's state flag and shape
+    // are restored on every return
     struct Restore {
         dnsgpu* h;
-        int ps, nblk;
+        int ps, nblk, w;
         ~Restore()
         {
             h->poststream = ps;
-            h->step_nblk = nblk;
+            (void)set_step_shape(h, w, nblk);
         }
-    } restore{h, h->poststream, h->step_nblk};
+    } restore{h, h->poststream, h->step_nblk, h->step_w};
     h->poststream = 0;
     const int wpcs[] = {2, 3, 4, 6, 8, 12, 16, 20, 24, 32};
-    int best = h->step_nblk;
-    double best_t = 1e30;
-    int last = -1;
-    for (int wpc : wpcs) {
-        int nb = (h->ncu * wpc + h->step_w - 1) / h->step_w;
-        nb = std::max(1, std::min(nb, std::min(h->step_nblk_max, h->step_resident)));
-        if (nb == last) continue;
-        last = nb;
-        h->step_nblk = nb;
-        // one untimed launch, then trial_steps timed ones; buffers are not flipped
-        for (int r = 0; r <= trial_steps; r++) {
-            if (r == 1) HIPCHK(hipEventRecord(h->ev0, h->stream));
-            with_storage(h->c.precision, [&](auto t) { enqueue_step_kernel<decltype(t)>(h, true); });
-            HIPCHK(hipGetLastError());
-        }
-        HIPCHK(hipEventRecord(h->ev1, h->stream));
-        HIPCHK(hipEventSynchronize(h->ev1));
-        float ms = 0.f;
-        HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
-        const double t = ms / trial_steps;
-        if (t < best_t) {
-            best_t = t;
-            best = nb;
+    struct Cand {
+        TuneShape s;
+        double t;
+    };
+    std::vector<Cand> cands;
+    for (int W = 2; W <= step_waves(h->g.Z); W++) {
+        int last = -1;
+        for (int wpc : wpcs) {
+            int rc = set_step_shape(h, W, (h->ncu * wpc + W - 1) / W);
+            if (rc) return rc;
+            if (h->step_nblk == last || h->step_w != W) continue;
+            last = h->step_nblk;
+            double t = 0.0;
+            rc = time_shape(h, trial_steps, &t);
+            if (rc) return rc;
+            cands.push_back({TuneShape{h->step_nblk, W}, t});
         }
     }
-    restore.nblk = best;
+    if (cands.empty()) return set_err(DNSGPU_EHIP, "autotune: no launch shape");
+    std::sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.t < b.t; });
+    const int nfin = std::min<int>(3, (int)cands.size());
+    for (int i = 0; i < nfin; i++) cands[i].t = 0.0;
+    for (int round = 0; round < 2; round++)
+        for (int i = 0; i < nfin; i++) {
+            int rc = set_step_shape(h, cands[i].s.waves, cands[i].s.blocks);
+            if (rc) return rc;
+            double t = 0.0;
+            rc = time_shape(h, trial_steps, &t);
+            if (rc) return rc;
+            cands[i].t += t;
+        }
+    int bi = 0;
+    for (int i = 1; i < nfin; i++)
+        if (cands[i].t < cands[bi].t) bi = i;
+    const TuneShape best = cands[bi].s;
+    restore.nblk = best.blocks;
+    restore.w = best.waves;
     tune_store(key, best);
-    if (best_blocks) *best_blocks = best;
-    if (best_ms) *best_ms = best_t;
+    if (best_blocks) *best_blocks = best.blocks;
+    if (best_ms) *best_ms = cands[bi].t / 2;
+    return 0;
+}
+
+extern "C" int dnsgpu_step_shape(const dnsgpu* h, int* blocks, int* waves_per_block)
+{
+    if (!h) return set_err(DNSGPU_EARG, "null handle");
+    if (blocks) *blocks = h->step_nblk;
+    if (waves_per_block) *waves_per_block = h->step_w;
     return 0;
 }
 

@@ -421,6 +421,12 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
     // masked reloads wait for the pulls whose registers they overwrite);
     // in FP16 3.5% faster at 768x384x192 but 3.4% slower in the ensemble.
     constexpr bool kReadSide = std::is_same<T, float>::value;
+    // Every store of the step is non-temporal: no kernel of the step reads
+    // the destination back, so its lines are not kept in the L2, which then
+    // holds the pulled lines a chunk shares with the next one (each e_z = +-1
+    // segment's line across a 64-row chunk boundary).  Measured: 3-5% in every
+    // precision at 768x384x192 and 1536x768x384 (riblets), and larger grids
+    // stop losing to refetches.
     const gbyte* __restrict__ s8 = (const gbyte*)A.src;
     gbyte* __restrict__ d8 = (gbyte*)A.dst;
     // byte addresses of the 3x3 neighbourhood's column blocks in the source
@@ -565,9 +571,9 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
                 const bool inside = EZ[a] == 0 ? true : (EZ[a] > 0 ? zt < Z : zt >= 0);
                 // x + e_a is not a fluid node (a band node or a wall row)
                 const bool nb_solid = a == 0 ? solid_own : solid_or_wall(bw[1 - EX[a]][1 - EY[a]], zt, Z);
-                if (!solid_own || nb_solid) *(GLOBAL_AS T*)(dp[1][1] + a * zbc + zoff) = v[a];
+                if (!solid_own || nb_solid) __builtin_nontemporal_store(v[a], (GLOBAL_AS T*)(dp[1][1] + a * zbc + zoff));
                 if (a != 0 && !solid_own && nb_solid && inside) {
-                    *(GLOBAL_AS T*)(dp[1 - EX[a]][1 - EY[a]] + (OPP[a] * zbc + EZ[a] * (int)sizeof(T)) + zoff) = v[a];
+                    __builtin_nontemporal_store(v[a], (GLOBAL_AS T*)(dp[1 - EX[a]][1 - EY[a]] + (OPP[a] * zbc + EZ[a] * (int)sizeof(T)) + zoff));
                     if constexpr (EX[a] != 0) mom_node -= 2.0 * (double)EX[a] * fpost(ac);
                 }
             });
@@ -593,7 +599,7 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
             asm volatile("" : "+v"(zo));
             static_for<0, Q>([&](auto ac) {
                 constexpr int a = decltype(ac)::value;
-                *(GLOBAL_AS T*)(dp[1][1] + a * zbc + zo) = v[a];
+                __builtin_nontemporal_store(v[a], (GLOBAL_AS T*)(dp[1][1] + a * zbc + zo));
             });
         }
         if (!solid_own) {

@@ -223,17 +223,24 @@ int dnsgpu_tavg_get(dnsgpu *h, dns_tavg *out, double *profiles);
  * whole sequence. */
 int dnsgpu_time_steps(dnsgpu *h, int nsteps, int with_stats, double *ms_step_kernels, double *ms_wall);
 
-/* Choose the step kernel's grid (blocks resident on the device) by timing
- * trial_steps launches of each candidate on this handle's own state and
- * masks.  The state is not advanced: the trials read the current buffer and
- * write the other one without flipping them, so call it before a restart's
- * first checkpoint is written (the second checkpoint half of a freshly
- * loaded state lives in the other buffer).  Results are cached per
- * (grid, precision, texture) in the process and, when the environment
- * variable DNSGPU_TUNE_CACHE names a file, across processes.  No reference
- * counterpart (the reference has no device to tune).  best_blocks and
- * best_ms may be NULL. */
+/* Choose the step kernel's launch shape by timing trial_steps launches of
+ * each candidate on this handle's own state and masks: the waves per block
+ * (each wave one column at a time; 2 to 4, as the per-wave plane sums fit
+ * the LDS: the block shape) and the blocks of the grid (2 to 32 waves per
+ * CU), then the three fastest again, alternating.  The state is not
+ * advanced: the trials read the current buffer and write the other one
+ * without flipping them, so call it before a restart's first checkpoint is
+ * written (the second checkpoint half of a freshly loaded state lives in
+ * the other buffer).  Results are cached per (grid, precision, texture) in
+ * the process and, when the environment variable DNSGPU_TUNE_CACHE names a
+ * file, across processes.  No reference counterpart (the reference's
+ * blocking is fixed, definitions.h:117).  best_blocks and best_ms may be
+ * NULL. */
 int dnsgpu_autotune(dnsgpu *h, int trial_steps, int *best_blocks, double *best_ms);
+
+/* The step kernel's current launch shape: blocks of the grid and waves per
+ * block (either pointer may be NULL).  No reference counterpart. */
+int dnsgpu_step_shape(const dnsgpu *h, int *blocks, int *waves_per_block);
 
 /* Device bytes held by the handle. */
 size_t dnsgpu_device_bytes(const dnsgpu *h);
