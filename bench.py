@@ -256,7 +256,7 @@ DRAG = {
     "drag_retau180_grooves_768x384x192": dict(base="retau180_768x384x192", texture=("grooves", 32, 16),
                                               steps=20000, avg_from=10000),
     "drag_retau395_riblets_1536x768x384": dict(base="retau395_1536x768x384", texture=("riblets", 16, 8),
-                                               steps=4000, avg_from=2000),
+                                               steps=20000, avg_from=10000),
 }
 
 
@@ -267,7 +267,70 @@ def stripes(X, Y, period, slip):
     return np.ascontiguousarray(np.broadcast_to(row[None, :], (X, Y)))
 
 
+def drag_run(name, prec, textured, steps, avg_from, batch=500):
+    """One run of a drag workload: the smooth (textured=False) or textured
+    channel at the workload's constant force from the seeded turbulent
+    field, averaged over [avg_from, steps): the time averages, the mean
+    profile, and the batch means (batch steps each) of the bulk velocity
+    and the wall stress."""
+    import numpy as np
+    d = DRAG[name]
+    w = WORKLOADS[d["base"]]
+    omega, F, ub, nu = case_params(w)
+    X, Y, Z = w["X"], w["Y"], w["Z"]
+    ch = dnsgpu.Channel(X, Y, Z, Rebs=ub * Z / nu, ub=ub, CFL=0.1, TSTR=0, forcing=dnsgpu.CONST_F,
+                        force=F, precision=prec, omega=omega)
+    tag = f"{PNAME[prec]} {'textured' if textured else 'smooth'}"
+    try:
+        kind, a, b = d["texture"]
+        if textured and kind == "grooves":
+            m = stripes(X, Y, a, b)
+            ch.set_walls(m, m)
+        elif textured:
+            ch.set_solid_bands(*([riblet_bands(X, Y, a, b)] * 2))
+        ch.init_field(dnsgpu.INIT_TURBULENT, uc=w["utau"], amp=0.1, seed=2, nmodes=8)
+        ch.autotune(trial_steps=3)
+        t0 = time.perf_counter()
+        done = 0
+        while done < avg_from:
+            k = min(2000, avg_from - done)
+            ch.step(k, cntr0=done, logs=False)
+            done += k
+            print(f"# {tag}: {done} steps, {time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
+        ch.tavg_reset()
+        ub_b, st_b = [], []
+        while done < steps:
+            k = min(batch, steps - done)
+            logs = ch.step(k, cntr0=done, logs=True)
+            ch.tavg_sample()
+            ub_b.append(float(np.mean(logs["flowrate"])) / (Y * Z))
+            st_b.append(float(np.mean(logs["stress"])))
+            done += k
+            if done % 2000 == 0 or done == steps:
+                print(f"# {tag}: {done} steps, {time.perf_counter() - t0:.0f} s", file=sys.stderr, flush=True)
+        wall = time.perf_counter() - t0
+        tv = ch.tavg(profiles=True)
+    finally:
+        ch.close()
+    tv["profiles"] = tv["profiles"].tolist()
+    return {"tavg": tv, "wall_s": wall, "Ub_batches": ub_b, "stress_batches": st_b, "batch_steps": batch,
+            "steps": steps, "avg_from": avg_from}
+
+
+def _batch_stats(x):
+    import numpy as np
+    x = np.asarray(x, dtype=float)
+    n = len(x)
+    return float(x.mean()), (float(x.std(ddof=1) / np.sqrt(n)) if n > 1 else float("nan")), n
+
+
 def run_drag(name, args):
+    """Drag reduction of a textured channel against the smooth one at the
+    same constant body force, FP64 / FP32 / FP16 storage from the same
+    seeded field, averaged over [avg_from, steps).  With --dr-dir each
+    finished run is kept as a file there and runs already there are read
+    back, so the six runs can be made in several processes; --dr-runs
+    selects the runs of this process."""
     import numpy as np
     d = DRAG[name]
     w = WORKLOADS[d["base"]]
@@ -275,73 +338,86 @@ def run_drag(name, args):
     X, Y, Z = w["X"], w["Y"], w["Z"]
     steps = args.dr_steps or d["steps"]
     avg_from = min(args.dr_avg_from or d["avg_from"], steps - 500)
+    want = None
+    if args.dr_runs:
+        want = {(p, t) for p, t in (r.split(":") for r in args.dr_runs.split(","))}
     runs = {}
     for prec in (dnsgpu.FP64, dnsgpu.FP32, dnsgpu.FP16):
         for textured in (False, True):
-            ch = dnsgpu.Channel(X, Y, Z, Rebs=ub * Z / nu, ub=ub, CFL=0.1, TSTR=0, forcing=dnsgpu.CONST_F,
-                                force=F, precision=prec, omega=omega)
-            try:
-                kind, a, b = d["texture"]
-                if textured and kind == "grooves":
-                    m = stripes(X, Y, a, b)
-                    ch.set_walls(m, m)
-                elif textured:
-                    ch.set_solid_bands(*([riblet_bands(X, Y, a, b)] * 2))
-                ch.init_field(dnsgpu.INIT_TURBULENT, uc=w["utau"], amp=0.1, seed=2, nmodes=8)
-                ch.autotune(trial_steps=3)
-                t0 = time.perf_counter()
-                ch.step(avg_from, cntr0=0, logs=False)
-                ch.tavg_reset()
-                done = avg_from
-                while done < steps:
-                    k = min(500, steps - done)
-                    ch.step(k, cntr0=done, logs=True)
-                    ch.tavg_sample()
-                    done += k
-                wall = time.perf_counter() - t0
-                tv = ch.tavg(profiles=True)
-            finally:
-                ch.close()
-            runs[(prec, textured)] = (tv, wall)
-            print(f"# {PNAME[prec]} {'textured' if textured else 'smooth'}: "
-                  f"Ub={tv['Ub']:.6e} {wall:.1f} s", file=sys.stderr, flush=True)
+            key = (PNAME[prec], "textured" if textured else "smooth")
+            path = os.path.join(args.dr_dir, f"{name}.{key[0]}.{key[1]}.json") if args.dr_dir else None
+            if path and os.path.exists(path):
+                with open(path) as fh:
+                    r = json.load(fh)
+                if r["steps"] == steps and r["avg_from"] == avg_from:
+                    runs[(prec, textured)] = r
+                    continue
+            if want is not None and key not in want:
+                continue
+            r = drag_run(name, prec, textured, steps, avg_from)
+            runs[(prec, textured)] = r
+            if path:
+                os.makedirs(args.dr_dir, exist_ok=True)
+                with open(path, "w") as fh:
+                    json.dump(r, fh)
+            print(f"# {key[0]} {key[1]}: Ub={r['tavg']['Ub']:.6e} {r['wall_s']:.1f} s", file=sys.stderr, flush=True)
 
-    def dr(prec):
-        return dnsgpu.drag_reduction(runs[(prec, False)][0], runs[(prec, True)][0], dnsgpu.CONST_F)["DR_Ub"]
-
-    t64 = runs[(dnsgpu.FP64, True)][0]
-    u64 = t64["profiles"][0]
-    ut64 = np.sqrt(abs(t64["stress"]))
-    slip64 = 0.5 * (t64["u_slip_bottom"] + t64["u_slip_top"])
-    nodes = X * Y * Z
+    h = Z / 2.0
+    utau_cfg = np.sqrt(F * h)
     out = {
         "metric": "MLUPS + % HBM roofline (FP32 vs FP64 D3Q19); ensemble solves/h + p99 latency",
-        "value": round(nodes * steps / runs[(dnsgpu.FP64, True)][1] / 1e6, 1),
-        "unit": "MLUPS",
-        "n_gpus": 1, "steps": steps, "warmup": 0,
-        "ms_per_step": round(runs[(dnsgpu.FP64, True)][1] / steps * 1e3, 4),
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "unit": "MLUPS", "n_gpus": 1, "steps": steps, "warmup": 0, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
         "config": {"workload": name, "grid_xdv_ydv_zdv": [X, Y, Z], "texture": "%s %d %d" % d["texture"],
                    "forcing": "constant body force", "averaging_window": [avg_from, steps],
+                   "window_eddy_turnovers": round((steps - avg_from) * utau_cfg / h, 3),
+                   "run_eddy_turnovers": round(steps * utau_cfg / h, 3),
                    "drag_reduction": "1 - (Ub_smooth / Ub_textured)^2 at constant force",
+                   "sampling_error": "standard error of 500-step batch means of Ub (batches of a developing "
+                                     "flow are correlated: a lower bound)",
                    "parallelism": "single GPU"},
-        "fp64": {"drag_reduction_percent": round(dr(dnsgpu.FP64), 4), "u_tau_wall_stress": ut64,
-                 "slip_velocity": slip64},
-        "Ub": {f"{PNAME[p]}_{'textured' if t else 'smooth'}": runs[(p, t)][0]["Ub"] for (p, t) in runs},
-        "wall_s": {f"{PNAME[p]}_{'textured' if t else 'smooth'}": round(runs[(p, t)][1], 2) for (p, t) in runs},
+        "runs_present": sorted(f"{PNAME[p]}_{'textured' if t else 'smooth'}" for (p, t) in runs),
     }
-    for prec in (dnsgpu.FP32, dnsgpu.FP16):
-        t = runs[(prec, True)][0]
-        ut = np.sqrt(abs(t["stress"]))
-        slip = 0.5 * (t["u_slip_bottom"] + t["u_slip_top"])
-        out[PNAME[prec] + "_shifted"] = {
-            "drag_reduction_percent": round(dr(prec), 4),
-            "drag_reduction_error_points": round(abs(dr(prec) - dr(dnsgpu.FP64)), 4),
-            "mean_profile_max_rel_error": float(np.abs(t["profiles"][0] - u64).max() / np.abs(u64).max()),
-            "u_tau_wall_stress_rel_error": abs(ut - ut64) / ut64,
-            "slip_velocity": slip,
-            "slip_velocity_rel_error": abs(slip - slip64) / abs(slip64) if slip64 else None,
-        }
+    if (dnsgpu.FP64, True) in runs:
+        r64 = runs[(dnsgpu.FP64, True)]
+        out["value"] = round(X * Y * Z * steps / r64["wall_s"] / 1e6, 1)
+        out["ms_per_step"] = round(r64["wall_s"] / steps * 1e3, 4)
+    out["Ub"] = {f"{PNAME[p]}_{'textured' if t else 'smooth'}": r["tavg"]["Ub"] for (p, t), r in runs.items()}
+    out["wall_s"] = {f"{PNAME[p]}_{'textured' if t else 'smooth'}": round(r["wall_s"], 2) for (p, t), r in runs.items()}
+
+    def dr_of(prec):
+        if (prec, False) not in runs or (prec, True) not in runs:
+            return None
+        u0, s0, _ = _batch_stats(runs[(prec, False)]["Ub_batches"])
+        u1, s1, n = _batch_stats(runs[(prec, True)]["Ub_batches"])
+        dr = dnsgpu.drag_reduction(runs[(prec, False)]["tavg"], runs[(prec, True)]["tavg"], dnsgpu.CONST_F)["DR_Ub"]
+        err = 200.0 * (u0 / u1) ** 2 * np.sqrt((s0 / u0) ** 2 + (s1 / u1) ** 2)
+        return dr, err, n
+
+    base = dr_of(dnsgpu.FP64)
+    t64 = runs.get((dnsgpu.FP64, True), {}).get("tavg")
+    for prec in (dnsgpu.FP64, dnsgpu.FP32, dnsgpu.FP16):
+        r = dr_of(prec)
+        if r is None:
+            continue
+        key = "fp64" if prec == dnsgpu.FP64 else PNAME[prec] + "_shifted"
+        t = runs[(prec, True)]["tavg"]
+        ent = {"drag_reduction_percent": round(r[0], 4), "sampling_error_points": round(r[1], 4),
+               "batches": r[2], "u_tau_wall_stress": float(np.sqrt(abs(t["stress"]))),
+               "slip_velocity": 0.5 * (t["u_slip_bottom"] + t["u_slip_top"])}
+        if prec != dnsgpu.FP64 and base is not None and t64 is not None:
+            u64 = np.asarray(t64["profiles"][0])
+            ut64 = np.sqrt(abs(t64["stress"]))
+            slip64 = 0.5 * (t64["u_slip_bottom"] + t64["u_slip_top"])
+            ent.update({
+                "drag_reduction_error_points": round(abs(r[0] - base[0]), 4),
+                "error_over_sampling_error": round(abs(r[0] - base[0]) / base[1], 3) if base[1] > 0 else None,
+                "mean_profile_max_rel_error": float(np.abs(np.asarray(t["profiles"][0]) - u64).max()
+                                                    / np.abs(u64).max()),
+                "u_tau_wall_stress_rel_error": float(abs(np.sqrt(abs(t["stress"])) - ut64) / ut64),
+                "slip_velocity_rel_error": (abs(ent["slip_velocity"] - slip64) / abs(slip64)) if slip64 else None,
+            })
+        out[key] = ent
     return out
 
 
@@ -513,6 +589,8 @@ def main():
     ap.add_argument("--sweep-fp16", action="store_true", help=f"{SWEEP}: FP16-shifted storage")
     ap.add_argument("--dr-steps", type=int, default=0, help="drag workloads: steps per run")
     ap.add_argument("--dr-avg-from", type=int, default=0, help="drag workloads: first averaged step")
+    ap.add_argument("--dr-runs", default="", help="drag workloads: runs of this process, e.g. fp64:smooth,fp64:textured")
+    ap.add_argument("--dr-dir", default="", help="drag workloads: directory keeping each finished run")
     ap.add_argument("--z-sweep", action="store_true", help="step-kernel GB/s against Z (see run_z_sweep)")
     ap.add_argument("--z-list", default="64,96,128,192,384")
     ap.add_argument("--z-prec", default="fp64,fp32,fp16")
@@ -552,10 +630,12 @@ def main():
     tr = measured_traffic(args.workload) or {}
 
     def traffic_of(p):
-        # the counters of the grid this run's autotuner chose, when that grid
-        # was profiled, else those of the profiled run's own choice
+        # the counters of the launch shape (blocks x threads) this run's
+        # autotuner chose, when that shape was profiled, else those of the
+        # profiled run's own choice
         ent = tr.get(f"{PNAME[p]}_stats") or {}
-        by = (ent.get("by_blocks") or {}).get(str(runs[p]["step_blocks"]))
+        shape = f"{runs[p]['step_blocks']}x{64 * runs[p]['step_waves_per_block']}"
+        by = (ent.get("by_shape") or {}).get(shape)
         return (by or ent).get("traffic_bytes_per_launch")
 
     traffic = {p: traffic_of(p) for p in runs}
