@@ -61,6 +61,7 @@ int ilog2(int v)
 
 struct dnsens {
     int precision;
+    int device;  // the device current at create; members live on it
     size_t budget;
     size_t used;
     int flags;
@@ -115,6 +116,7 @@ extern "C" int dnsens_create(int precision, size_t hbm_budget, int flags, dnsens
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&e->ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
         return fail(set_err(DNSGPU_EHIP, "device query failed"));
+    e->device = dev;
     if (hbm_budget == 0) {
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) != hipSuccess) return fail(set_err(DNSGPU_EHIP, "hipMemGetInfo failed"));
@@ -202,6 +204,8 @@ extern "C" int dnsens_add(dnsens* e, dnsgpu* h)
     if (h->c.precision != e->precision)
         return set_err(DNSGPU_EARG, "member precision %d differs from the ensemble's %d", h->c.precision,
                        e->precision);
+    if (h->device != e->device)
+        return set_err(DNSGPU_EARG, "member on device %d, the ensemble on device %d", h->device, e->device);
     if (find_member(e, h) >= 0) return set_err(DNSGPU_EARG, "handle is already a member");
     if (h->owner) return set_err(DNSGPU_EARG, "handle is a member of another ensemble");
     if ((int)e->mem.size() >= DNSENS_MAX_MEMBERS)
@@ -277,6 +281,13 @@ extern "C" int dnsens_remove(dnsens* e, dnsgpu* h)
 extern "C" int dnsens_count(const dnsens* e) { return e ? (int)e->mem.size() : 0; }
 extern "C" size_t dnsens_bytes_used(const dnsens* e) { return e ? e->used : 0; }
 extern "C" size_t dnsens_budget(const dnsens* e) { return e ? e->budget : 0; }
+
+extern "C" int dnsens_bind_thread(const dnsens* e)
+{
+    if (!e) return set_err(DNSGPU_EARG, "null argument");
+    HIPCHK(hipSetDevice(e->device));
+    return 0;
+}
 
 template <typename T>
 static int step_occupancy(bool shl, size_t lds, int* per_cu)

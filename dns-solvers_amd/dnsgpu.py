@@ -29,7 +29,7 @@ EXPORTS = (
     "dnsgpu_autotune", "dnsgpu_tavg_reset", "dnsgpu_tavg_sample", "dnsgpu_tavg_get",
     "dnsgpu_footprint", "dnsgpu_fields_noalloc", "dnsgpu_step_shape",
     "dnsens_create", "dnsens_destroy", "dnsens_member_bytes", "dnsens_footprint", "dnsens_add", "dnsens_remove",
-    "dnsens_count", "dnsens_bytes_used", "dnsens_budget", "dnsens_step", "dnsens_info", "dnsens_rows",
+    "dnsens_count", "dnsens_bytes_used", "dnsens_budget", "dnsens_bind_thread", "dnsens_step", "dnsens_info", "dnsens_rows",
     "dnsens_autotune", "dnsens_time_steps", "dnsens_time_kernels", "dnsens_grid",
 )
 

@@ -36,7 +36,7 @@ int dnsio_format_row(int which, const dns_steplog *row, char *buf, int len);
 int dnsio_write_turbfld(const char *dir, long ts, const double *stats, int zd);
 
 /* output_twoD.c:34-46 from the wall-normal column ux (zd values, walls 0). */
-int dnsio_write_twoD(const char *dir, const double *lb_ux, int zd, double ub);
+int dnsio_write_twoD(const char *dir, const double *lb_ux, int zd, double ubulk);
 
 /* Time averages: tavg-fld.000.NNN (9 x zd doubles, the turb-fld layout of
  * the time-averaged profiles) and tavg-summary.txt (u_tau, Re_tau, Cf,

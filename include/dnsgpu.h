@@ -40,6 +40,27 @@
 #include <stddef.h>
 #include <stdint.h>
 
+/* The case macros of definitions.h:30-37 share their names with the fields
+ * of dns_case: the header sets them aside while it declares, and restores
+ * them at its end, so it can be included after definitions.h.  Code that
+ * sees those macros fills a dns_case with dns_case_make. */
+#pragma push_macro("Rebs")
+#pragma push_macro("ub")
+#pragma push_macro("xdv")
+#pragma push_macro("ydv")
+#pragma push_macro("zdv")
+#pragma push_macro("TSTR")
+#pragma push_macro("TEND")
+#pragma push_macro("CFL")
+#undef Rebs
+#undef ub
+#undef xdv
+#undef ydv
+#undef zdv
+#undef TSTR
+#undef TEND
+#undef CFL
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -73,6 +94,26 @@ typedef struct dns_case {
     int precision;      /* enum dnsgpu_precision */
     double omega;       /* 1/tau; 0 => (3/Rebs)*ub*zdv+0.5 inverted (dns-const-q-xycut.c:216-217) */
 } dns_case;
+
+/* A case from definitions.h's macros, positionally (dns_case_make(xdv, ydv,
+ * zdv, Rebs, ub, CFL, TSTR) in a file that includes definitions.h):
+ * constant flow rate, FP64, omega from Rebs. */
+static inline dns_case dns_case_make(int nx, int ny, int nz, double rebs, double ubulk, double cfl, int tstr)
+{
+    dns_case c;
+    c.xdv = nx;
+    c.ydv = ny;
+    c.zdv = nz;
+    c.Rebs = rebs;
+    c.ub = ubulk;
+    c.CFL = cfl;
+    c.TSTR = tstr;
+    c.forcing = DNSGPU_CONST_Q;
+    c.force = 0.0;
+    c.precision = DNSGPU_FP64;
+    c.omega = 0.0;
+    return c;
+}
 
 /* One row of the seven per-step logs, computed from the state before the
  * step's collision: encalc.c:125-138 and wshearstr.c:133-154. */
@@ -297,6 +338,11 @@ int dnsens_remove(dnsens *e, dnsgpu *h);
 int dnsens_count(const dnsens *e);
 size_t dnsens_bytes_used(const dnsens *e);
 size_t dnsens_budget(const dnsens *e);
+/* Makes the ensemble's device (the one current at dnsens_create) current on
+ * the calling thread, so that handles this thread creates can join it:
+ * dnsens_add refuses a member created on another device.  For a thread that
+ * prepares members while another steps the ensemble. */
+int dnsens_bind_thread(const dnsens *e);
 
 /* Advance every member by nsteps logged steps; returns when they are done.
  * A member whose force or log row turned NaN is flagged in its info (and
@@ -341,4 +387,13 @@ int dnsens_grid(const dnsens *e, int *blocks, int *tiles);
 #ifdef __cplusplus
 }
 #endif
+
+#pragma pop_macro("Rebs")
+#pragma pop_macro("ub")
+#pragma pop_macro("xdv")
+#pragma pop_macro("ydv")
+#pragma pop_macro("zdv")
+#pragma pop_macro("TSTR")
+#pragma pop_macro("TEND")
+#pragma pop_macro("CFL")
 #endif

@@ -21,6 +21,7 @@ Run in a container that has the reference sources and MPICH:
 import hashlib
 import os
 import sys
+import tempfile
 
 import numpy as np
 
@@ -109,12 +110,14 @@ def run_case(name, c, workroot):
 def main():
     if not refrun.available():
         sys.exit("reference sources or mpirun not available")
-    work = os.path.join(refrun.REPO, "_work", "golden_runs")
-    for name, c in CASES.items():
-        out = run_case(name, c, work)
-        path = os.path.join(HERE, name + ".npz")
-        np.savez_compressed(path, **out)
-        print(name, os.path.getsize(path), "bytes")
+    # the reference's run directories (restart pair, logs, fields) are
+    # scratch: only the fixtures under tests/golden are kept
+    with tempfile.TemporaryDirectory(prefix="dns_golden_") as work:
+        for name, c in CASES.items():
+            out = run_case(name, c, work)
+            path = os.path.join(HERE, name + ".npz")
+            np.savez_compressed(path, **out)
+            print(name, os.path.getsize(path), "bytes")
 
 
 if __name__ == "__main__":
