@@ -216,7 +216,10 @@ extern "C" int dnsens_add(dnsens* e, dnsgpu* h)
     const size_t need = dnsens_member_bytes(h);
     if (e->used + need > e->budget)
         return set_err(DNSGPU_ENOMEM, "member needs %zu bytes, %zu of %zu in use", need, e->used, e->budget);
-    // the handle's own queued work first
+    // the ensemble's FP16 step bounces at bands on the write side: a state
+    // from the two-row single-case step first gets the pushed values; then
+    // the handle's own queued work
+    if (int rc = band_pushes(h)) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
     Member m;
     memset(&m, 0, sizeof m);

@@ -68,6 +68,23 @@ static int scratch_address(int dev, void** out)
 }
 
 
+// Two rows per lane: FP16 storage and an even number of rows.
+static bool step_rows_ok(int precision, int Z) { return precision == DNSGPU_FP16 && Z >= 2 && Z % 2 == 0; }
+
+// Blocks of the step kernel per CU at W waves per block and `rows` rows per
+// lane (the statistics variant with bands: the largest register set).
+static hipError_t step_occupancy(int precision, int Z, int W, int rows, int* per)
+{
+    const size_t lds = step_lds_bytes(Z, W, true);
+    if (rows == 2)
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, step_kernel<_Float16, false, true, true, 2>,
+                                                            kWave * W, lds);
+    return with_storage(precision, [&](auto t) {
+        return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, step_kernel<decltype(t), false, true, true>,
+                                                            kWave * W, lds);
+    });
+}
+
 template <typename K>
 static int occupancy_blocks(K kernel, int block, int* out)
 {
@@ -150,15 +167,52 @@ static int reduce_partials(dnsgpu* h, bool stats, int nblk)
     return 0;
 }
 
-// The step kernel alone, on the handle's stream.
+int band_pushes(dnsgpu* h)
+{
+    if (!h->rs_state) return 0;
+    h->rs_state = 0;
+    if (!h->has_bands || h->poststream) return 0;
+    const long long n = (long long)h->g.ncols * h->g.Z;
+    const int nb = (int)std::min<long long>((n + 255) / 256, 8LL * h->ncu);
+    int rc = with_storage(h->c.precision, [&](auto t) {
+        using T = decltype(t);
+        hipLaunchKernelGGL(band_push_kernel<T>, dim3(nb), dim3(256), 0, h->stream, (T*)h->buf[h->cur], h->bands,
+                           h->g.X, h->g.Y, h->g.Z);
+        return 0;
+    });
+    if (rc) return rc;
+    HIPCHK(hipGetLastError());
+    return 0;
+}
+
+// The step kernel alone, on the handle's stream.  A write-side band step
+// (FP16 one row per lane) of a state from a read-side one first gets the
+// pushed values it pulls (band_pushes).
 template <typename T>
 static void enqueue_step_kernel(dnsgpu* h, bool stats)
 {
+    if constexpr (std::is_same<T, _Float16>::value) {
+        if (h->step_rows != 2) (void)band_pushes(h);
+    }
     const void* src = h->buf[h->cur];
     void* dst = h->buf[1 - h->cur];
     StepArgs A = step_args(h, src, dst, h->step_nblk);
     dim3 grid(h->step_nblk), block(kWave * h->step_w);
     const size_t lds = step_lds_bytes(h->g.Z, h->step_w, stats);
+    if constexpr (std::is_same<T, _Float16>::value) {
+        if (h->step_rows == 2 && !h->poststream) {
+#define STEP_R2(S, B) hipLaunchKernelGGL((step_kernel<T, false, S, B, 2>), grid, block, lds, h->stream, A)
+            if (h->has_bands) {
+                if (stats) STEP_R2(true, true);
+                else STEP_R2(false, true);
+            } else {
+                if (stats) STEP_R2(true, false);
+                else STEP_R2(false, false);
+            }
+#undef STEP_R2
+            return;
+        }
+    }
 #define STEP_VARIANT(F, S, B) hipLaunchKernelGGL((step_kernel<T, F, S, B>), grid, block, lds, h->stream, A)
     if (h->has_bands) {
         if (h->poststream) {
@@ -183,6 +237,8 @@ static void enqueue_step_kernel(dnsgpu* h, bool stats)
 template <typename T>
 static int launch_step(dnsgpu* h, bool stats, LogRow* row, long long cntr)
 {
+    // the two-row FP16 step bounces at bands on the read side
+    const bool rs = std::is_same<T, _Float16>::value && h->step_rows == 2 && !h->poststream && h->has_bands;
     enqueue_step_kernel<T>(h, stats);
     HIPCHK(hipGetLastError());
     int rc = reduce_partials(h, stats, h->step_nblk);
@@ -192,6 +248,7 @@ static int launch_step(dnsgpu* h, bool stats, LogRow* row, long long cntr)
     HIPCHK(hipGetLastError());
     h->cur = 1 - h->cur;
     h->poststream = 0;
+    h->rs_state = rs ? 1 : 0;
     return 0;
 }
 
@@ -260,6 +317,9 @@ static int plan_handle(const dns_case* c, dnsgpu* h)
     h->step_w = step_waves(g.Z);
     const char* swe = getenv("DNSGPU_STEP_WAVES");
     if (swe) h->step_w = std::max(1, std::min(atoi(swe), h->step_w));
+    h->step_rows = 1;
+    const char* sre = getenv("DNSGPU_STEP_ROWS");
+    if (sre && atoi(sre) == 2 && step_rows_ok(c->precision, g.Z)) h->step_rows = 2;
     h->esize = storage_bytes(c->precision);
     // omega and the constants exactly as dns-const-q-xycut.c:211-217,
     // wshearstr.c:24, encalc.c:20 evaluate them.
@@ -302,12 +362,7 @@ static int plan_handle(const dns_case* c, dnsgpu* h)
     int sb = (ncu * waves_per_cu + h->step_w - 1) / h->step_w;
     // never more blocks than can be resident at once (the grid loops)
     int per = 0;
-    const size_t lds_stats = step_lds_bytes(g.Z, h->step_w, true);
-    const hipError_t oe = with_storage(c->precision, [&](auto t) {
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, step_kernel<decltype(t), false, true, true>,
-                                                            kWave * h->step_w, lds_stats);
-    });
-    HIPCHK(oe);
+    HIPCHK(step_occupancy(c->precision, g.Z, h->step_w, h->step_rows, &per));
     h->step_resident = std::max(1, per) * ncu;
     sb = std::min(sb, h->step_resident);
     const char* sbe = getenv("DNSGPU_STEP_BLOCKS");
@@ -413,6 +468,7 @@ extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
     // the pop buffers start at rest
     h->cur = 0;
     h->poststream = 1;
+    h->rs_state = 0;
     if (hipStreamSynchronize(h->stream) != hipSuccess) return fail(set_err(DNSGPU_EHIP, "sync failed"));
     count_slip_columns(h, nullptr, nullptr);
     rc = dnsgpu_init_rest(h);
@@ -454,7 +510,7 @@ struct TuneKey {
 // threads): guarded
 // a tuned launch shape: blocks of the grid and waves per block
 struct TuneShape {
-    int blocks, waves;
+    int blocks, waves, rows;
 };
 std::vector<std::pair<TuneKey, TuneShape>> g_tune_cache;
 std::mutex g_tune_mutex;
@@ -474,7 +530,8 @@ bool tune_lookup(const TuneKey& k, TuneShape* out)
     TuneKey r;
     TuneShape sh;
     bool found = false;
-    while (fscanf(f, "%d %d %d %d %d %d %d", &r.X, &r.Y, &r.Z, &r.prec, &r.bands, &sh.blocks, &sh.waves) == 7)
+    while (fscanf(f, "%d %d %d %d %d %d %d %d", &r.X, &r.Y, &r.Z, &r.prec, &r.bands, &sh.blocks, &sh.waves,
+                  &sh.rows) == 8)
         if (r == k) {
             *out = sh;
             found = true;
@@ -492,23 +549,22 @@ void tune_store(const TuneKey& k, TuneShape sh)
     if (!path) return;
     FILE* f = fopen(path, "a");
     if (!f) return;
-    fprintf(f, "%d %d %d %d %d %d %d\n", k.X, k.Y, k.Z, k.prec, k.bands, sh.blocks, sh.waves);
+    fprintf(f, "%d %d %d %d %d %d %d %d\n", k.X, k.Y, k.Z, k.prec, k.bands, sh.blocks, sh.waves, sh.rows);
     fclose(f);
 }
 }  // namespace
 
-// Sets the step kernel's launch shape (W waves per block, nblk blocks)
-// within what the LDS, the device and the partial buffers allow.
-static int set_step_shape(dnsgpu* h, int W, int nblk)
+// Sets the step kernel's launch shape (W waves per block, nblk blocks,
+// rows per lane) within what the LDS, the device and the partial buffers
+// allow.
+static int set_step_shape(dnsgpu* h, int W, int nblk, int rows)
 {
     W = std::max(1, std::min(W, step_waves(h->g.Z)));
+    rows = rows == 2 && step_rows_ok(h->c.precision, h->g.Z) ? 2 : 1;
     int per = 0;
-    const hipError_t oe = with_storage(h->c.precision, [&](auto t) {
-        return hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, step_kernel<decltype(t), false, true, true>,
-                                                            kWave * W, step_lds_bytes(h->g.Z, W, true));
-    });
-    HIPCHK(oe);
+    HIPCHK(step_occupancy(h->c.precision, h->g.Z, W, rows, &per));
     h->step_w = W;
+    h->step_rows = rows;
     h->step_resident = std::max(1, per) * h->ncu;
     const int cols = std::max(1, (h->g.ncols + W - 1) / W);
     h->step_nblk = std::max(1, std::min(nblk, std::min(std::min(h->step_nblk_max, h->step_resident), cols)));
@@ -539,7 +595,7 @@ extern "C" int dnsgpu_autotune(dnsgpu* h, int trial_steps, int* best_blocks, dou
     const TuneKey key{h->g.X, h->g.Y, h->g.Z, h->c.precision, h->has_bands};
     TuneShape cached;
     if (tune_lookup(key, &cached)) {
-        int rc = set_step_shape(h, cached.waves, cached.blocks);
+        int rc = set_step_shape(h, cached.waves, cached.blocks, cached.rows);
         if (rc) return rc;
         if (best_blocks) *best_blocks = h->step_nblk;
         if (best_ms) *best_ms = -1.0;
@@ -551,13 +607,13 @@ extern "C" int dnsgpu_autotune(dnsgpu* h, int trial_steps, int* best_blocks, dou
     // are restored on every return
     struct Restore {
         dnsgpu* h;
-        int ps, nblk, w;
+        int ps, nblk, w, rows;
         ~Restore()
         {
             h->poststream = ps;
-            (void)set_step_shape(h, w, nblk);
+            (void)set_step_shape(h, w, nblk, rows);
         }
-    } restore{h, h->poststream, h->step_nblk, h->step_w};
+    } restore{h, h->poststream, h->step_nblk, h->step_w, h->step_rows};
     h->poststream = 0;
     const int wpcs[] = {2, 3, 4, 6, 8, 12, 16, 20, 24, 32};
     struct Cand {
@@ -565,26 +621,28 @@ extern "C" int dnsgpu_autotune(dnsgpu* h, int trial_steps, int* best_blocks, dou
         double t;
     };
     std::vector<Cand> cands;
-    for (int W = 2; W <= step_waves(h->g.Z); W++) {
-        int last = -1;
-        for (int wpc : wpcs) {
-            int rc = set_step_shape(h, W, (h->ncu * wpc + W - 1) / W);
-            if (rc) return rc;
-            if (h->step_nblk == last || h->step_w != W) continue;
-            last = h->step_nblk;
-            double t = 0.0;
-            rc = time_shape(h, trial_steps, &t);
-            if (rc) return rc;
-            cands.push_back({TuneShape{h->step_nblk, W}, t});
+    const int max_rows = step_rows_ok(h->c.precision, h->g.Z) ? 2 : 1;
+    for (int R = 1; R <= max_rows; R++)
+        for (int W = 2; W <= step_waves(h->g.Z); W++) {
+            int last = -1;
+            for (int wpc : wpcs) {
+                int rc = set_step_shape(h, W, (h->ncu * wpc + W - 1) / W, R);
+                if (rc) return rc;
+                if (h->step_nblk == last || h->step_w != W) continue;
+                last = h->step_nblk;
+                double t = 0.0;
+                rc = time_shape(h, trial_steps, &t);
+                if (rc) return rc;
+                cands.push_back({TuneShape{h->step_nblk, W, R}, t});
+            }
         }
-    }
     if (cands.empty()) return set_err(DNSGPU_EHIP, "autotune: no launch shape");
     std::sort(cands.begin(), cands.end(), [](const Cand& a, const Cand& b) { return a.t < b.t; });
     const int nfin = std::min<int>(3, (int)cands.size());
     for (int i = 0; i < nfin; i++) cands[i].t = 0.0;
     for (int round = 0; round < 2; round++)
         for (int i = 0; i < nfin; i++) {
-            int rc = set_step_shape(h, cands[i].s.waves, cands[i].s.blocks);
+            int rc = set_step_shape(h, cands[i].s.waves, cands[i].s.blocks, cands[i].s.rows);
             if (rc) return rc;
             double t = 0.0;
             rc = time_shape(h, trial_steps, &t);
@@ -597,17 +655,31 @@ extern "C" int dnsgpu_autotune(dnsgpu* h, int trial_steps, int* best_blocks, dou
     const TuneShape best = cands[bi].s;
     restore.nblk = best.blocks;
     restore.w = best.waves;
+    restore.rows = best.rows;
     tune_store(key, best);
     if (best_blocks) *best_blocks = best.blocks;
     if (best_ms) *best_ms = cands[bi].t / 2;
     return 0;
 }
 
-extern "C" int dnsgpu_step_shape(const dnsgpu* h, int* blocks, int* waves_per_block)
+extern "C" int dnsgpu_set_step_shape(dnsgpu* h, int blocks, int waves_per_block, int rows_per_lane)
+{
+    if (!h) return set_err(DNSGPU_EARG, "null handle");
+    NOT_MEMBER(h);
+    if (blocks < 1 || waves_per_block < 1 || waves_per_block > step_waves(h->g.Z))
+        return set_err(DNSGPU_EARG, "launch shape %d blocks x %d waves (1..%d waves at zdv %d)", blocks,
+                       waves_per_block, step_waves(h->g.Z), h->g.Z);
+    if (rows_per_lane != 1 && !(rows_per_lane == 2 && step_rows_ok(h->c.precision, h->g.Z)))
+        return set_err(DNSGPU_EARG, "%d rows per lane (2 needs FP16 storage and an even zdv)", rows_per_lane);
+    return set_step_shape(h, waves_per_block, blocks, rows_per_lane);
+}
+
+extern "C" int dnsgpu_step_shape(const dnsgpu* h, int* blocks, int* waves_per_block, int* rows_per_lane)
 {
     if (!h) return set_err(DNSGPU_EARG, "null handle");
     if (blocks) *blocks = h->step_nblk;
     if (waves_per_block) *waves_per_block = h->step_w;
+    if (rows_per_lane) *rows_per_lane = h->step_rows;
     return 0;
 }
 
@@ -869,6 +941,7 @@ static int launch_init(dnsgpu* h, int kind, const dns_init* spec)
     HIPCHK(hipGetLastError());
     h->cur = 0;
     h->poststream = 1;
+    h->rs_state = 0;
     int rc = refresh_force(h);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -927,6 +1000,7 @@ extern "C" int dnsgpu_load_aos(dnsgpu* h, int x0, int nx, const double* ftemp, c
     HIPCHK(hipStreamSynchronize(h->stream));
     h->cur = 0;
     h->poststream = 1;
+    h->rs_state = 0;
     return 0;
 }
 
@@ -936,6 +1010,7 @@ extern "C" int dnsgpu_commit_load(dnsgpu* h)
     NOT_MEMBER(h);
     h->cur = 0;
     h->poststream = 1;
+    h->rs_state = 0;
     int rc = refresh_force(h);
     if (rc) return rc;
     HIPCHK(hipStreamSynchronize(h->stream));

@@ -32,6 +32,7 @@ struct dnsgpu {
     void* buf[2];
     int cur;         // buffer holding the current state
     int poststream;  // 1: buf[cur] holds post-stream populations (after load/init)
+    int rs_state;    // 1: buf[cur] is from a read-side band step (solid slots hold no pushes)
     int8_t* mbot;
     int8_t* mtop;
     uint32_t* wword;
@@ -56,6 +57,7 @@ struct dnsgpu {
     int ring_cap;
     int nblk;       // blocks of the chunk-mapped kernels (moments, statistics)
     int step_w;     // waves per block of the step kernel
+    int step_rows;  // rows per lane of the step kernel (2: FP16 storage, even Z)
     int step_nblk;  // blocks of the step kernel
     int ncu;
     int step_nblk_max;  // capacity of the partial buffers for the step grid
@@ -98,6 +100,9 @@ inline int storage_bytes(int precision)
 
 // Step arguments of the handle's current state (src) into dst.
 __attribute__((visibility("hidden"))) StepArgs step_args(dnsgpu* h, const void* src, void* dst, int nblk);
+// Fills the solid band slots of buf[cur] with the pushed values a
+// write-side step pulls, when buf[cur] came from a read-side band step.
+__attribute__((visibility("hidden"))) int band_pushes(dnsgpu* h);
 // Finisher arguments of the handle (row: log row slot or null).
 __attribute__((visibility("hidden"))) FinalizeArgs fin_args(dnsgpu* h, bool stats, LogRow* row,
                                                                 long long cntr, int from_moments);

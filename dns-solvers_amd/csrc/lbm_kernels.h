@@ -610,6 +610,228 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
     }
 }
 
+// Two rows per lane (FP16 storage, even Z): the wave walks its column in
+// 128-row chunks, lane l holding rows z = 128 ch + 2 l (low half) and z + 1
+// (high half) of every direction in one dword, so every load and store
+// moves 4 bytes per lane (256 bytes per wave instruction), the bytes an
+// FP32 access moves, where the one-row form moves 2.  The pulls of the
+// e_z = +1 and -1 directions are the dwords at rows (z - 1, z) and
+// (z + 1, z + 2): 2-byte aligned, a form the amdhsa target's unaligned
+// access mode serves as one dword load.  At row 0 the e_z = +1 pull reads
+// the last row of the previous direction segment of the same column block
+// (directions 5, 11, 14, 15, 18 all follow another segment) and at row
+// Z - 1 the e_z = -1 pull the first row of the next one (directions 6, 12,
+// 13, 16, 17 all precede another): in bounds, and replaced by the wall
+// rule.  Bands bounce on the read side (the values the write side gives to
+// every fluid node).  Each node's arithmetic is the one-row form's.
+typedef uint32_t __attribute__((aligned(2))) u32_a2;
+
+__device__ __forceinline__ _Float16 half_of(uint32_t w, int hi)
+{
+    return __builtin_bit_cast(_Float16, (uint16_t)(hi ? w >> 16 : w & 0xFFFFu));
+}
+
+__device__ __forceinline__ uint32_t with_half(uint32_t w, int hi, _Float16 h)
+{
+    const uint32_t b = (uint32_t)__builtin_bit_cast(uint16_t, h);
+    return hi ? (w & 0xFFFFu) | (b << 16) : (w & 0xFFFF0000u) | b;
+}
+
+__device__ __forceinline__ uint32_t with_half16(uint32_t w, int hi, uint16_t b)
+{
+    return hi ? (w & 0xFFFFu) | ((uint32_t)b << 16) : (w & 0xFFFF0000u) | (uint32_t)b;
+}
+
+template <bool STATS, bool BANDS>
+__device__ __forceinline__ void step_column_r2(const StepArgs& A, int col, int lane, double G, const double* shl,
+                                               double* mine, StepAcc& acc)
+{
+    using T = _Float16;
+    const int Z = A.g.Z;
+    const int nzc2 = (Z + 2 * kWave - 1) / (2 * kWave);
+    // the lane whose high half is row Z - 1
+    const int ctop = (Z - 2) / (2 * kWave), ltop = ((Z - 2) % (2 * kWave)) / 2;
+    const gbyte* __restrict__ s8 = (const gbyte*)A.src;
+    gbyte* __restrict__ d8 = (gbyte*)A.dst;
+    const gbyte* sp[3][3];
+    gbyte* dst;
+    {
+        const ColNbr cr = load_uniform(reinterpret_cast<const ColNbr*>(A.cnbr) + col);
+        const uint32_t qzb = (uint32_t)(Q * Z * (int)sizeof(T));
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) sp[i][j] = s8 + (unsigned long long)cr.w[3 * i + j] * qzb;
+        dst = d8 + (unsigned long long)cr.w[4] * qzb;
+    }
+    const int zb = Z * (int)sizeof(T);
+    const uint32_t ww = load_uniform_u32(A.wword + col);
+    uint32_t bw[3][3];
+    int maxhb = 0, maxht = 0;
+    if constexpr (BANDS) {
+        const BandNbr nb = load_uniform(reinterpret_cast<const BandNbr*>(A.bnbr) + col);
+#pragma unroll
+        for (int i = 0; i < 3; i++)
+#pragma unroll
+            for (int j = 0; j < 3; j++) bw[i][j] = nb.w[3 * i + j];
+        maxhb = (int)(nb.w[9] & 0xFFFFu);
+        maxht = (int)(nb.w[9] >> 16);
+    }
+    for (int ch = 0; ch < nzc2; ch++) {
+        const int z = ch * 2 * kWave + 2 * lane;
+        if (z >= Z) break;
+        const bool bot_lane = ch == 0 && lane == 0;
+        const bool top_lane = ch == ctop && lane == ltop;
+        const int zoff = z * (int)sizeof(T);
+        int zbc = zb;
+        asm volatile("" : "+s"(zbc));
+        uint32_t v[Q];
+        static_for<0, Q>([&](auto ac) {
+            constexpr int a = decltype(ac)::value;
+            const gbyte* p = sp[EX[a] + 1][EY[a] + 1] + a * zbc;
+            if constexpr (EZ[a] == 0)
+                v[a] = *(const GLOBAL_AS uint32_t*)(p + zoff);
+            else
+                v[a] = *(const GLOBAL_AS u32_a2*)(p + (zoff - EZ[a] * (int)sizeof(T)));
+        });
+        if (bot_lane) {
+            static_for<0, Q>([&](auto ac) {
+                constexpr int a = decltype(ac)::value;
+                if constexpr (EZ[a] == 1) {
+                    const bool spec = (ww >> wall_bit<a>()) & 1u;
+                    const gbyte* p = spec ? sp[EX[a] + 1][EY[a] + 1] + MIR[a] * zbc : sp[1][1] + OPP[a] * zbc;
+                    v[a] = with_half16(v[a], 0, *(const GLOBAL_AS uint16_t*)p);
+                }
+            });
+        }
+        if (top_lane) {
+            static_for<0, Q>([&](auto ac) {
+                constexpr int a = decltype(ac)::value;
+                if constexpr (EZ[a] == -1) {
+                    const bool spec = (ww >> wall_bit<a>()) & 1u;
+                    const gbyte* p = spec ? sp[EX[a] + 1][EY[a] + 1] + MIR[a] * zbc : sp[1][1] + OPP[a] * zbc;
+                    v[a] = with_half16(v[a], 1, *(const GLOBAL_AS uint16_t*)(p + (zbc - (int)sizeof(T))));
+                }
+            });
+        }
+        bool slow = false, solid0 = false, solid1 = false;
+        if constexpr (BANDS) {
+            const int z0 = ch * 2 * kWave, z1 = min(z0 + 2 * kWave, Z);
+            slow = (maxhb > 0 && z0 - 1 < maxhb) || (maxht > 0 && z1 >= Z - maxht);
+            solid0 = band_solid(bw[1][1], z, Z);
+            solid1 = band_solid(bw[1][1], z + 1, Z);
+            if (slow) {
+                // read-side halfway bounce-back per node (see step_column)
+                static_for<1, Q>([&](auto ac) {
+                    constexpr int a = decltype(ac)::value;
+                    const uint32_t bn = bw[1 + EX[a]][1 + EY[a]];
+                    const bool c0 = band_solid(bn, z - EZ[a], Z), c1 = band_solid(bn, z + 1 - EZ[a], Z);
+                    if (c0 || c1) {
+                        const uint32_t own = *(const GLOBAL_AS uint32_t*)(sp[1][1] + OPP[a] * zbc + zoff);
+                        const uint32_t m = (c0 ? 0xFFFFu : 0u) | (c1 ? 0xFFFF0000u : 0u);
+                        v[a] = (v[a] & ~m) | (own & m);
+                    }
+                });
+            }
+        }
+        // the two nodes of the lane, row z then row z + 1 (one at a time:
+        // unrolled, the two collisions interleave and double the registers)
+#pragma unroll 1
+        for (int n = 0; n < 2; n++) {
+            const int zn = z + n;
+            const bool solid_own = n ? solid1 : solid0;
+            T hv[Q];
+#pragma unroll
+            for (int a = 0; a < Q; a++) hv[a] = half_of(v[a], n);
+            float drho, u0, u1, u2, jx;
+            collide_stored(hv, (float)A.omega, (float)G, drho, u0, u1, u2, jx);
+#pragma unroll
+            for (int a = 0; a < Q; a++) v[a] = with_half(v[a], n, hv[a]);
+            const double jx_raw = (double)jx, mass_node = (double)drho;
+            if constexpr (STATS) {
+                if (!solid_own) {
+                    const double du = (double)u0 - shl[zn], dv = (double)u1 - shl[Z + zn],
+                                 dw = (double)u2 - shl[2 * Z + zn];
+                    mine[0 * Z + zn] += du;
+                    mine[1 * Z + zn] += dv;
+                    mine[2 * Z + zn] += dw;
+                    mine[3 * Z + zn] += mass_node;
+                    acc.sq += (du * du + dv * dv) + dw * dw;
+                }
+            }
+            auto fpost = [&](auto ac) -> double {
+                constexpr int a = decltype(ac)::value;
+                return WD[a] + (double)Store<T>::to_g(hv[a]);
+            };
+            using I11 = std::integral_constant<int, 11>;
+            using I12 = std::integral_constant<int, 12>;
+            using I13 = std::integral_constant<int, 13>;
+            using I14 = std::integral_constant<int, 14>;
+            double mom_node = jx_raw + G;
+            if (n == 0 && bot_lane)
+                mom_node = mom_node - (((ww >> 10) & 1u) ? 2.0 * fpost(I12{}) : 0.0) +
+                           (((ww >> 11) & 1u) ? 2.0 * fpost(I13{}) : 0.0);
+            if (n == 1 && top_lane)
+                mom_node = mom_node - (((ww >> 12) & 1u) ? 2.0 * fpost(I11{}) : 0.0) +
+                           (((ww >> 13) & 1u) ? 2.0 * fpost(I14{}) : 0.0);
+            if constexpr (BANDS) {
+                if (slow) {
+                    static_for<1, Q>([&](auto ac) {
+                        constexpr int a = decltype(ac)::value;
+                        if constexpr (EX[a] != 0) {
+                            if (!solid_own && band_solid(bw[1 - EX[a]][1 - EY[a]], zn + EZ[a], Z))
+                                mom_node -= 2.0 * (double)EX[a] * fpost(ac);
+                        }
+                    });
+                }
+            }
+            if (!solid_own) {
+                if constexpr (STATS) acc.jx += jx_raw;
+                acc.mass += mass_node;
+                acc.mom += mom_node;
+            }
+        }
+        uint32_t zo = (uint32_t)zoff;
+        asm volatile("" : "+v"(zo));
+        static_for<0, Q>([&](auto ac) {
+            constexpr int a = decltype(ac)::value;
+            __builtin_nontemporal_store(v[a], (GLOBAL_AS uint32_t*)(dst + a * zbc + zo));
+        });
+    }
+}
+
+// A state written by a read-side band step leaves the slots of solid band
+// nodes holding the solid nodes' own values; a write-side step pulls, at a
+// fluid node x whose upstream node t = x - e_b is solid, t's slot b, which
+// a write-side step fills with x's post-collision opp(b).  This pass gives
+// every such slot that value (x's own slot opp(b), which both sides store):
+// after it the state serves either side.  Each solid slot has one writer
+// (t's slot b is pushed only from t + e_b) and no fluid slot is written,
+// so the pass has no read-write overlap.  One thread per fluid node next to
+// a band.
+template <typename T>
+static __global__ __launch_bounds__(256) void band_push_kernel(T* __restrict__ st, const uint32_t* __restrict__ bands,
+                                                               int X, int Y, int Z)
+{
+    const long long total = (long long)X * Y * Z;
+    for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < total;
+         i += (long long)gridDim.x * blockDim.x) {
+        const int z = (int)(i % Z);
+        const int col = (int)(i / Z);
+        const Column c = make_column_lane(col, X, Y, Z);
+        if (band_solid(bands[col], z, Z)) continue;
+        static_for<1, Q>([&](auto ac) {
+            constexpr int a = decltype(ac)::value;
+            // t = x + e_a, the node x pushes a toward
+            const int tx = EX[a] == 1 ? c.xp : (EX[a] == -1 ? c.xm : c.x);
+            const int ty = EY[a] == 1 ? c.yp : (EY[a] == -1 ? c.ym : c.y);
+            const int tz = z + EZ[a];
+            if (tz >= 0 && tz < Z && band_solid(bands[tx * Y + ty], tz, Z))
+                st[((long long)(tx * Y + ty) * Q + OPP[a]) * Z + tz] = st[c.cb + (long long)a * Z + z];
+        });
+    }
+}
+
 // Deterministic reductions of one block: plane sums over the block's waves
 // in wave order, global sums by a fixed butterfly over each wave's lanes and
 // then over the waves in order; written as partial number `slot` of
@@ -652,7 +874,20 @@ __device__ __forceinline__ void step_block_reduce(double* red, const double* pla
 
 // The single-case step: the block's waves take columns round-robin over
 // the grid; one set of partials per block.
-template <typename T, bool FIRST, bool STATS, bool BANDS>
+template <typename T, bool FIRST, bool STATS, bool BANDS, int R>
+__device__ __forceinline__ void step_column_of(const StepArgs& A, int col, int lane, double G, const double* shl,
+                                               double* mine, StepAcc& acc)
+{
+    if constexpr (R == 2) {
+        static_assert(std::is_same<T, _Float16>::value && !FIRST, "two rows per lane: FP16 pull steps only");
+        step_column_r2<STATS, BANDS>(A, col, lane, G, shl, mine, acc);
+    } else {
+        step_column<T, FIRST, STATS, BANDS>(A, col, lane, G, shl, mine, acc);
+    }
+}
+
+// R: rows per lane (1, or 2 for FP16 storage with an even Z).
+template <typename T, bool FIRST, bool STATS, bool BANDS, int R = 1>
 static __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs A)
 {
     extern __shared__ double lds[];  // see step_lds_bytes
@@ -676,11 +911,13 @@ static __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs 
     const double G = *A.G;
     if (!STATS || shl_lds) {
         for (int it = blockIdx.x * W + wave; it < A.g.ncols; it += gridDim.x * W)
-            step_column<T, FIRST, STATS, BANDS>(A, __builtin_amdgcn_readfirstlane(it), lane, G, shl, mine, acc);
+            step_column_of<T, FIRST, STATS, BANDS, R>(A, __builtin_amdgcn_readfirstlane(it), lane, G, shl, mine,
+                                                      acc);
     } else {
         // the same loop with the shifts read from global memory
         for (int it = blockIdx.x * W + wave; it < A.g.ncols; it += gridDim.x * W)
-            step_column<T, FIRST, STATS, BANDS>(A, __builtin_amdgcn_readfirstlane(it), lane, G, A.shift, mine, acc);
+            step_column_of<T, FIRST, STATS, BANDS, R>(A, __builtin_amdgcn_readfirstlane(it), lane, G, A.shift,
+                                                      mine, acc);
     }
     step_block_reduce<STATS>(red, planes, acc, Z, W, wave, lane, A.part, A.gpart, A.nblk, blockIdx.x);
 }

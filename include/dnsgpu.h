@@ -267,8 +267,11 @@ int dnsgpu_time_steps(dnsgpu *h, int nsteps, int with_stats, double *ms_step_ker
 /* Choose the step kernel's launch shape by timing trial_steps launches of
  * each candidate on this handle's own state and masks: the waves per block
  * (each wave one column at a time; 2 to 4, as the per-wave plane sums fit
- * the LDS: the block shape) and the blocks of the grid (2 to 32 waves per
- * CU), then the three fastest again, alternating.  The state is not
+ * the LDS: the block shape), the blocks of the grid (2 to 32 waves per CU)
+ * and, for FP16 storage with an even zdv, the rows per lane (1, or 2 rows
+ * in one 4-byte access per direction: the cells per thread), then the
+ * three fastest again, alternating.  Every shape gives the same state:
+ * only the summation order of the logged sums depends on it.  The state is not
  * advanced: the trials read the current buffer and write the other one
  * without flipping them, so call it before a restart's first checkpoint is
  * written (the second checkpoint half of a freshly loaded state lives in
@@ -279,9 +282,15 @@ int dnsgpu_time_steps(dnsgpu *h, int nsteps, int with_stats, double *ms_step_ker
  * NULL. */
 int dnsgpu_autotune(dnsgpu *h, int trial_steps, int *best_blocks, double *best_ms);
 
-/* The step kernel's current launch shape: blocks of the grid and waves per
- * block (either pointer may be NULL).  No reference counterpart. */
-int dnsgpu_step_shape(const dnsgpu *h, int *blocks, int *waves_per_block);
+/* The step kernel's current launch shape: blocks of the grid, waves per
+ * block and rows per lane (any pointer may be NULL).  No reference
+ * counterpart. */
+int dnsgpu_step_shape(const dnsgpu *h, int *blocks, int *waves_per_block, int *rows_per_lane);
+/* Sets the launch shape by hand (blocks are capped at what is resident at
+ * once); rows_per_lane 2 needs FP16 storage and an even zdv.  The state
+ * and the results do not depend on the shape beyond the summation order of
+ * the logged sums.  No reference counterpart. */
+int dnsgpu_set_step_shape(dnsgpu *h, int blocks, int waves_per_block, int rows_per_lane);
 
 /* Device bytes held by the handle. */
 size_t dnsgpu_device_bytes(const dnsgpu *h);
