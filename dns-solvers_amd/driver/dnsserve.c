@@ -7,6 +7,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <pthread.h>
 #include <time.h>
 
 #include "dnscase.h"
@@ -214,6 +215,113 @@ static void retire(dnsens *e, resident *m, const dnsserve_request *req, dnsserve
     m->hist = NULL;
 }
 
+/* Requests are prepared (handle created, textured, initialised: dnsserve_open)
+ * by a thread of their own in arrival order while the GPU advances the
+ * resident solves, and are admitted to the ensemble at the next chunk
+ * boundary.  A request is prepared only when its footprint fits the budget
+ * together with the members and the requests already prepared. */
+typedef struct prep_state {
+    const dnsserve_request *req;
+    const dnsserve_config *cfg;
+    dnsserve_result *res;
+    const dnsens *e;
+    int n;
+    double t0;
+    size_t budget;
+    pthread_mutex_t mu;
+    pthread_cond_t cv;
+    size_t reserved;     /* footprints of the members and of the prepared requests */
+    int nready;          /* prepared, not yet admitted: ready[head .. head + nready) */
+    int ready_head;
+    int *ready_req;
+    dnsgpu **ready_h;
+    size_t *ready_fp;
+    int refused;         /* requests refused by the preparer (status -2) */
+    int prepared_all;    /* the preparer has handled every request */
+    int stop;            /* the sweep ends: the preparer returns */
+    int error;           /* an error of the preparer (a DNSGPU_E* code) */
+    char errmsg[512];    /* its message (the library's messages are per thread) */
+} prep_state;
+
+static void prep_fail(prep_state *p, int rc)
+{
+    pthread_mutex_lock(&p->mu);
+    p->error = rc;
+    snprintf(p->errmsg, sizeof p->errmsg, "%s", dnsgpu_last_error());
+    p->stop = 1;
+    pthread_cond_broadcast(&p->cv);
+    pthread_mutex_unlock(&p->mu);
+}
+
+static void *prep_main(void *arg)
+{
+    prep_state *p = arg;
+    const int brc = dnsens_bind_thread(p->e);
+    if (brc) {
+        prep_fail(p, brc);
+        return NULL;
+    }
+    for (int i = 0; i < p->n; i++) {
+        const dnsserve_request *r = p->req + i;
+        if (p->cfg->realtime) sleep_s(r->arrival - (now_s() - p->t0));
+        dns_case c;
+        dns_init init;
+        dnsserve_case(r, p->cfg, &c, &init);
+        const size_t fp = dnsens_footprint(&c);
+        pthread_mutex_lock(&p->mu);
+        p->res[i].arrival = p->cfg->realtime ? r->arrival : 0.0;
+        if (fp == 0 || fp > p->budget) { /* a case the library refuses, or larger than the budget */
+            p->res[i].status = -2;
+            p->res[i].finished = now_s() - p->t0;
+            p->refused++;
+            pthread_cond_broadcast(&p->cv);
+            pthread_mutex_unlock(&p->mu);
+            continue;
+        }
+        while (!p->stop && p->reserved + fp > p->budget) pthread_cond_wait(&p->cv, &p->mu);
+        if (p->stop) {
+            pthread_mutex_unlock(&p->mu);
+            break;
+        }
+        p->reserved += fp;
+        pthread_mutex_unlock(&p->mu);
+        dnsgpu *h = NULL;
+        int rc = dnsserve_open(r, p->cfg, &h);
+        pthread_mutex_lock(&p->mu);
+        if (rc == DNSGPU_ENOMEM && p->reserved > fp) {
+            /* the device is fuller than the budget says: retry when a solve retires */
+            const size_t before = p->reserved - fp;
+            p->reserved -= fp;
+            while (!p->stop && p->reserved >= before) pthread_cond_wait(&p->cv, &p->mu);
+            pthread_mutex_unlock(&p->mu);
+            if (p->stop) break;
+            i--;
+            continue;
+        }
+        if (rc) {
+            p->reserved -= fp;
+            pthread_mutex_unlock(&p->mu);
+            prep_fail(p, rc);
+            break;
+        }
+        /* charge what the member will hold (the same bytes as its footprint) */
+        const size_t held = dnsens_member_bytes(h);
+        p->reserved = p->reserved - fp + held;
+        const int slot = p->ready_head + p->nready;
+        p->ready_req[slot] = i;
+        p->ready_h[slot] = h;
+        p->ready_fp[slot] = held;
+        p->nready++;
+        pthread_cond_broadcast(&p->cv);
+        pthread_mutex_unlock(&p->mu);
+    }
+    pthread_mutex_lock(&p->mu);
+    p->prepared_all = 1;
+    pthread_cond_broadcast(&p->cv);
+    pthread_mutex_unlock(&p->mu);
+    return NULL;
+}
+
 int dnsserve_sweep(const dnsserve_request *req, int n, const dnsserve_config *cfg, dnsserve_result *res,
                    dnsserve_summary *sum)
 {
@@ -223,46 +331,65 @@ int dnsserve_sweep(const dnsserve_request *req, int n, const dnsserve_config *cf
     dnsens *e = NULL;
     int rc = dnsens_create(cfg->precision, cfg->budget, cfg->autotune ? DNSENS_AUTOTUNE : 0, &e);
     if (rc) return rc;
-    resident *live = calloc((size_t)(n > 0 ? n : 1), sizeof(resident));
-    int *queue = malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
-    if (!live || !queue) {
+    const int nn = n > 0 ? n : 1;
+    resident *live = calloc((size_t)nn, sizeof(resident));
+    prep_state p;
+    memset(&p, 0, sizeof p);
+    p.ready_req = malloc(sizeof(int) * (size_t)nn);
+    p.ready_h = malloc(sizeof(dnsgpu *) * (size_t)nn);
+    p.ready_fp = malloc(sizeof(size_t) * (size_t)nn);
+    if (!live || !p.ready_req || !p.ready_h || !p.ready_fp) {
         free(live);
-        free(queue);
+        free(p.ready_req);
+        free(p.ready_h);
+        free(p.ready_fp);
         dnsens_destroy(e);
         return DNSGPU_ENOMEM;
     }
+    p.req = req;
+    p.cfg = cfg;
+    p.res = res;
+    p.e = e;
+    p.n = n;
+    p.budget = dnsens_budget(e);
+    pthread_mutex_init(&p.mu, NULL);
+    pthread_cond_init(&p.cv, NULL);
     const int max_checks = cfg->cap / cfg->check + 2;
     const int wchecks = cfg->window / cfg->check;
-    int nlive = 0, qhead = 0, qtail = 0, next = 0, done = 0;
+    int nlive = 0, done = 0, retired = 0;
     double used_peak = 0.0;
     const double t0 = now_s();
+    p.t0 = t0;
     double busy = 0.0;
-    while (done < n) {
-        double t = now_s() - t0;
-        /* arrivals */
-        while (next < n && (!cfg->realtime || req[next].arrival <= t)) {
-            res[next].arrival = cfg->realtime ? req[next].arrival : 0.0;
-            queue[qtail++] = next++;
+    pthread_t th;
+    if (pthread_create(&th, NULL, prep_main, &p) != 0) {
+        rc = DNSGPU_EARG;
+        goto out_nothread;
+    }
+    while (1) {
+        /* admit what is prepared (at a chunk boundary); wait when nothing runs */
+        pthread_mutex_lock(&p.mu);
+        while (nlive == 0 && p.nready == 0 && !p.error && !(p.prepared_all && retired + p.refused >= n))
+            pthread_cond_wait(&p.cv, &p.mu);
+        done = retired + p.refused;
+        if (p.error) {
+            rc = p.error;
+            /* the library's message is the preparer thread's: repeat it here */
+            fprintf(stderr, "dnsserve: preparing a request failed: %s\n", p.errmsg);
+            pthread_mutex_unlock(&p.mu);
+            goto out;
         }
-        /* admission in arrival order while the head fits */
-        while (qhead < qtail) {
-            const int i = queue[qhead];
-            dns_case c;
-            dns_init init;
-            dnsserve_case(req + i, cfg, &c, &init);
-            const size_t fp = dnsens_footprint(&c);
-            if (fp == 0) { /* a case the library refuses */
-                res[i].status = -2;
-                res[i].finished = now_s() - t0;
-                qhead++;
-                done++;
-                continue;
-            }
-            if (dnsens_bytes_used(e) + fp > dnsens_budget(e)) break;
-            dnsgpu *h = NULL;
-            rc = dnsserve_open(req + i, cfg, &h);
-            if (rc == DNSGPU_ENOMEM) break;
-            if (rc) goto out;
+        if (nlive == 0 && p.nready == 0 && p.prepared_all && done >= n) {
+            pthread_mutex_unlock(&p.mu);
+            break;
+        }
+        int nadd = p.nready, head = p.ready_head;
+        p.ready_head += nadd;
+        p.nready = 0;
+        pthread_mutex_unlock(&p.mu);
+        for (int k = 0; k < nadd; k++) {
+            const int i = p.ready_req[head + k];
+            dnsgpu *h = p.ready_h[head + k];
             rc = dnsens_add(e, h);
             if (rc) {
                 dnsgpu_destroy(h);
@@ -280,21 +407,10 @@ int dnsserve_sweep(const dnsserve_request *req, int n, const dnsserve_config *cf
             if (cfg->verbose)
                 fprintf(stderr, "dnsserve: t=%.2fs admit %d (%dx%dx%d) resident=%d bytes=%zu\n", res[i].admitted, i,
                         req[i].xdv, req[i].ydv, req[i].zdv, nlive, dnsens_bytes_used(e));
-            qhead++;
             if ((double)dnsens_bytes_used(e) > used_peak) used_peak = (double)dnsens_bytes_used(e);
             if (nlive > sum->max_resident) sum->max_resident = nlive;
         }
-        if (nlive == 0) {
-            if (qhead < qtail) { /* the head does not fit an empty ensemble */
-                const int i = queue[qhead++];
-                res[i].status = -2;
-                res[i].finished = now_s() - t0;
-                done++;
-                continue;
-            }
-            if (next < n && cfg->realtime) sleep_s(req[next].arrival - (now_s() - t0));
-            continue;
-        }
+        if (nlive == 0) continue;
         const double tb = now_s();
         rc = dnsens_step(e, cfg->check);
         if (rc) goto out;
@@ -329,12 +445,17 @@ int dnsserve_sweep(const dnsserve_request *req, int n, const dnsserve_config *cf
             if (status == 2 && s + 1.0 >= cfg->cap) status = 0;
             if (status != 2) {
                 const int ri = m->req;
+                const size_t fp = dnsens_member_bytes(m->h);
                 retire(e, m, req, res, now_s() - t0, status);
+                pthread_mutex_lock(&p.mu);
+                p.reserved -= fp;
+                retired++;
+                pthread_cond_broadcast(&p.cv);
+                pthread_mutex_unlock(&p.mu);
                 if (cfg->verbose)
                     fprintf(stderr, "dnsserve: t=%.2fs done %d status=%d steps=%.0f u_tau=%.4e\n", res[ri].finished,
                             ri, status, res[ri].steps, res[ri].utau);
                 live[k] = live[--nlive];
-                done++;
             } else {
                 k++;
             }
@@ -344,7 +465,7 @@ int dnsserve_sweep(const dnsserve_request *req, int n, const dnsserve_config *cf
     rc = 0;
     {
         double first = 1e300, last = 0.0, node_steps = 0.0;
-        double *ttr = malloc(sizeof(double) * (size_t)(n > 0 ? n : 1));
+        double *ttr = malloc(sizeof(double) * (size_t)nn);
         int nt = 0;
         for (int i = 0; i < n; i++) {
             if (res[i].arrival < first) first = res[i].arrival;
@@ -373,6 +494,14 @@ int dnsserve_sweep(const dnsserve_request *req, int n, const dnsserve_config *cf
         sum->peak_bytes = used_peak;
     }
 out:
+    pthread_mutex_lock(&p.mu);
+    p.stop = 1;
+    pthread_cond_broadcast(&p.cv);
+    pthread_mutex_unlock(&p.mu);
+    pthread_join(th, NULL);
+    /* prepared requests never admitted */
+    for (int k = 0; k < p.nready; k++) dnsgpu_destroy(p.ready_h[p.ready_head + k]);
+out_nothread:
     for (int k = 0; k < nlive; k++) {
         if (live[k].h) {
             dnsens_remove(e, live[k].h);
@@ -381,7 +510,11 @@ out:
         free(live[k].hist);
     }
     free(live);
-    free(queue);
+    free(p.ready_req);
+    free(p.ready_h);
+    free(p.ready_fp);
+    pthread_mutex_destroy(&p.mu);
+    pthread_cond_destroy(&p.cv);
     dnsens_destroy(e);
     return rc;
 }

@@ -43,7 +43,7 @@ typedef struct dnsserve_request {
 } dnsserve_request;
 
 typedef struct dnsserve_config {
-    int precision;     /* DNSGPU_FP32 or DNSGPU_FP64 */
+    int precision;     /* DNSGPU_FP64, DNSGPU_FP32 or DNSGPU_FP16 */
     double ub;         /* lattice bulk velocity */
     int transient;     /* steps not averaged */
     int window;        /* convergence window (steps) */
@@ -100,7 +100,15 @@ void dnsserve_case(const dnsserve_request *r, const dnsserve_config *cfg, dns_ca
 /* Create, texture and initialise the handle of a request. */
 int dnsserve_open(const dnsserve_request *r, const dnsserve_config *cfg, dnsgpu **out);
 
-/* Serve n requests (sorted by arrival) through one ensemble. */
+/* Serve n requests (sorted by arrival) through one ensemble.  A thread of
+ * its own prepares the requests in arrival order (dnsserve_open: the handle
+ * created, textured and initialised on the ensemble's device) once each
+ * footprint fits the budget beside the resident and the already prepared
+ * solves; the calling thread admits the prepared ones at every check
+ * boundary, steps the ensemble and retires.  A request whose footprint
+ * exceeds the whole budget is refused (status -2).  Each solve's results
+ * do not depend on the mix, on the arrival times or on when it was
+ * admitted. */
 int dnsserve_sweep(const dnsserve_request *req, int n, const dnsserve_config *cfg, dnsserve_result *res,
                    dnsserve_summary *sum);
 

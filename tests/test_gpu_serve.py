@@ -97,15 +97,16 @@ def test_dnsserve_command_line(tmp_path):
 def test_texture_sweep_of_64_requests_alone_and_in_the_mix():
     """The configured sweep: 64 seeded requests (grids {128,256,512} x
     {64,128,256} x {64,96,128}, Re_tau in [90,180], grooves or riblets),
-    FP32-shifted storage, all resident together, with a short transient,
-    window and cap; a sample of the solves (the first groove, the first
-    riblet and the largest grid) served alone gives bitwise the same
-    results as in the mix of 64."""
+    FP32-shifted storage, all arriving at once and admitted as they are
+    prepared (the first solves run while the later ones are created), with
+    a short transient, window and cap; a sample of the solves (the first
+    groove, the first riblet and the largest grid) served alone gives
+    bitwise the same results as in the mix of 64."""
     print("case: serve the 64-request sweep, FP32, cap 400", flush=True)
     reqs = dnsserve.generate(64, seed=5, mean_gap=2.0)
     c = cfg(transient=200, window=200, check=100, cap=400, precision=dnsgpu.FP32, init_modes=8)
     mix, summ = dnsserve.sweep(reqs, c)
-    assert summ["completed"] == 64 and summ["max_resident"] == 64, [r["status"] for r in mix]
+    assert summ["completed"] == 64 and summ["max_resident"] >= 8, ([r["status"] for r in mix], summ)
     for r in mix:
         assert r["status"] in (0, 1) and 201 <= r["steps"] <= 401 and r["utau"] > 0, r
     tex = [r.texture for r in reqs]
@@ -116,3 +117,21 @@ def test_texture_sweep_of_64_requests_alone_and_in_the_mix():
         alone, _ = dnsserve.sweep(one, c)
         for k in KEYS:
             assert alone[0][k] == mix[i][k], (i, k)
+
+
+def test_request_larger_than_the_budget_is_refused():
+    """A request whose footprint exceeds the whole budget is refused (status
+    -2) when it comes up; the requests before and after it are served and
+    give the results they give alone."""
+    print("case: serve with one request above the budget", flush=True)
+    items = [SMALL[0], SMALL[2], SMALL[3]]
+    fp = [dnsgpu.member_footprint(r["xdv"], r["ydv"], r["zdv"], dnsgpu.FP32) for r in items]
+    budget = fp[0] + fp[2]
+    assert fp[1] > budget
+    res, summ = dnsserve.sweep(dnsserve.requests(items), cfg(budget=budget))
+    assert [r["status"] for r in res][1] == -2 and summ["completed"] == 2
+    for i in (0, 2):
+        assert res[i]["status"] in (0, 1)
+        alone, _ = dnsserve.sweep(dnsserve.requests([items[i]]), cfg())
+        for k in KEYS:
+            assert alone[0][k] == res[i][k], (i, k)
