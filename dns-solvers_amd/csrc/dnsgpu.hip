@@ -1017,6 +1017,20 @@ extern "C" int dnsgpu_commit_load(dnsgpu* h)
     return 0;
 }
 
+extern "C" int dnsgpu_set_force(dnsgpu* h, double force)
+{
+    if (!h) return set_err(DNSGPU_EARG, "null handle");
+    NOT_MEMBER(h);
+    if (h->c.forcing != DNSGPU_CONST_F)
+        return set_err(DNSGPU_EARG, "a constant-flow-rate case takes its force from its state (encalc.c:121)");
+    if (!std::isfinite(force)) return set_err(DNSGPU_EARG, "force %g not finite", force);
+    h->c.force = force;
+    // DevState.G: the force of the next step
+    HIPCHK(hipMemcpyAsync(&h->st->G, &force, sizeof force, hipMemcpyHostToDevice, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return 0;
+}
+
 extern "C" int dnsgpu_save_aos(dnsgpu* h, int x0, int nx, double* ftemp, double* f)
 {
     if (!h || !ftemp || !f) return set_err(DNSGPU_EARG, "null argument");

@@ -27,7 +27,7 @@ EXPORTS = (
     "dnsgpu_commit_load", "dnsgpu_save_aos", "dnsgpu_get_force", "dnsgpu_step",
     "dnsgpu_statistics", "dnsgpu_fields", "dnsgpu_time_steps", "dnsgpu_device_bytes",
     "dnsgpu_autotune", "dnsgpu_tavg_reset", "dnsgpu_tavg_sample", "dnsgpu_tavg_get",
-    "dnsgpu_footprint", "dnsgpu_fields_noalloc", "dnsgpu_step_shape", "dnsgpu_set_step_shape",
+    "dnsgpu_footprint", "dnsgpu_fields_noalloc", "dnsgpu_step_shape", "dnsgpu_set_step_shape", "dnsgpu_set_force",
     "dnsens_create", "dnsens_destroy", "dnsens_member_bytes", "dnsens_footprint", "dnsens_add", "dnsens_remove",
     "dnsens_count", "dnsens_bytes_used", "dnsens_budget", "dnsens_bind_thread", "dnsens_step", "dnsens_info", "dnsens_rows",
     "dnsens_autotune", "dnsens_time_steps", "dnsens_time_kernels", "dnsens_grid",
@@ -124,6 +124,7 @@ def load():
         L.dnsgpu_step_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                         ctypes.POINTER(ctypes.c_int)]
         L.dnsgpu_set_step_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.dnsgpu_set_force.argtypes = [vp, ctypes.c_double]
         L.dnsgpu_time_steps.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp]
         L.dnsgpu_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), dp]
         L.dnsgpu_tavg_reset.argtypes = [vp]
@@ -323,6 +324,9 @@ class Channel:
         b, w, r = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
         _chk(load().dnsgpu_step_shape(self.h, ctypes.byref(b), ctypes.byref(w), ctypes.byref(r)))
         return (b.value, w.value, r.value) if rows else (b.value, w.value)
+
+    def set_force(self, force):
+        _chk(load().dnsgpu_set_force(self.h, force))
 
     def set_step_shape(self, blocks, waves, rows=1):
         _chk(load().dnsgpu_set_step_shape(self.h, blocks, waves, rows))

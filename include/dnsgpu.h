@@ -201,6 +201,11 @@ int dnsgpu_save_aos(dnsgpu *h, int x0, int nx, double *ftemp, double *f);
 
 /* Current V.force (the force the next step will use); force.NNNN. */
 int dnsgpu_get_force(dnsgpu *h, double *force);
+/* Sets the force of a DNSGPU_CONST_F case from the next step on: the
+ * reference's V.force as xcomputations takes it (xcomputations.c:29), for a
+ * caller that computes the force itself (encalc.c:121 on its own host
+ * copy).  DNSGPU_EARG for a constant-flow-rate case. */
+int dnsgpu_set_force(dnsgpu *h, double force);
 
 /* Advance nsteps time steps: for each step, encalc + wshearstr + one fused
  * collide-stream (dns-const-q-xycut.c:487-488,518).  cntr0 is the loop

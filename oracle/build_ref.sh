@@ -16,6 +16,10 @@
 #   PSLIP = literal word PSLIP to enable slip posts (slip_posts_init.c)
 # REF_XDIM (environment, default 1): MPI ranks along x (definitions.h:58,
 # the rank count the binary must be started with; XDV a multiple of it)
+# REF_SHIM=1 (environment): also link <name>/run-dns-gpu, the same sources
+# with dns-solvers_amd/driver/xcomputations_gpu.c in place of
+# xcomputations.c, against the built dns-solvers_amd/lib/libdnsgpu.so: the
+# reference's own main on the GPU step (one rank)
 set -euo pipefail
 REF=${DNS_REFERENCE_SRC:-/root/reference/DNS-solver/src}
 HERE=$(cd "$(dirname "$0")" && pwd)
@@ -47,3 +51,19 @@ MPICH_CC=gcc "$MPICC" -O3 -fcommon -w -I"$OUT" -I"$REF" -I- "$REF"/*.c -lm -o "$
   || { cat "$OUT/build.log" >&2; exit 1; }
 mv "$OUT/run-dns.tmp" "$OUT/run-dns"
 echo "$OUT/run-dns"
+if [ "${REF_SHIM:-0}" = "1" ]; then
+  REPO=$(cd "$HERE/.." && pwd)
+  SRCS=$(ls "$REF"/*.c | grep -v '/xcomputations\.c$')
+  # gcc with the MPI wrapper's flags spelled out: the HIP runtime needs the
+  # system C++ runtime, which the wrapper's library directory (an older one)
+  # would shadow, at link time and through the run path
+  MPIDIR=$(dirname "$(dirname "$MPICC")")
+  SYSLIB=/usr/lib/x86_64-linux-gnu
+  gcc -O3 -fcommon -w -I"$OUT" -I"$REPO/include" -I"$REF" -I- -I"$MPIDIR/include" $SRCS \
+    "$REPO/dns-solvers_amd/driver/xcomputations_gpu.c" -L"$REPO/dns-solvers_amd/lib" -ldnsgpu \
+    -Wl,-rpath-link,"$SYSLIB" -L"$MPIDIR/lib" -lmpi -lm -Wl,--disable-new-dtags \
+    -Wl,-rpath,"$SYSLIB:$MPIDIR/lib" -Wl,-rpath,'$ORIGIN/../../../dns-solvers_amd/lib' \
+    -o "$OUT/run-dns-gpu.tmp" 2> "$OUT/build-gpu.log" || { cat "$OUT/build-gpu.log" >&2; exit 1; }
+  mv "$OUT/run-dns-gpu.tmp" "$OUT/run-dns-gpu"
+  echo "$OUT/run-dns-gpu"
+fi
