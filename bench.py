@@ -82,7 +82,7 @@ def run_precision(wname, prec, steps, warmup):
         else:
             ch.init_field(dnsgpu.INIT_TURBULENT, uc=w["utau"], amp=0.1, seed=2, nmodes=8)
         blocks, _ = ch.autotune(trial_steps=5)
-        _, waves = ch.step_shape()
+        _, waves, rows = ch.step_shape(rows=True)
         ch.step(warmup, cntr0=0, logs=True)
         t0 = time.perf_counter()
         logs = ch.step(steps, cntr0=warmup, logs=True)  # returns after the stream drained
@@ -111,6 +111,7 @@ def run_precision(wname, prec, steps, warmup):
         "finite": ok,
         "step_blocks": blocks,
         "step_waves_per_block": waves,
+        "step_rows_per_lane": rows,
         "stress_last": float(logs["stress"][-1]),
     }
 
@@ -567,6 +568,118 @@ def run_z_sweep(args):
             "data": "synthetic", "z_sweep": out}
 
 
+def run_rows_sweep(args):
+    """FP16 storage, one against two rows per lane (rows z and z + 1 in one
+    4-byte access per direction, the autotuner's cells-per-thread axis):
+    step-kernel time (statistics variant, HIP events, --steps launches) at
+    every waves-per-block x grid shape of the autotuner's search, each shape
+    timed twice in alternation, on the Re_tau 395 riblet grid and on smooth
+    grids of about 56.6 M nodes with zdv = 192, 96, 160, 128 (the last
+    chunk full, a quarter full, half full, full)."""
+    cases = [("1536x768x384 riblets s16 h8", 1536, 768, 384, (16, 8)), ("768x384x192 smooth", 768, 384, 192, None),
+             ("1536x384x96 smooth", 1536, 384, 96, None), ("920x384x160 smooth", 920, 384, 160, None),
+             ("1152x384x128 smooth", 1152, 384, 128, None)]
+    out = []
+    for name, X, Y, Z, rib in cases:
+        ch = dnsgpu.Channel(X, Y, Z, Rebs=3000.0, ub=0.1, CFL=0.1, forcing=dnsgpu.CONST_F, force=1e-6,
+                            precision=dnsgpu.FP16)
+        try:
+            if rib:
+                hb = riblet_bands(X, Y, *rib)
+                ch.set_solid_bands(hb, hb)
+            ch.init_field(dnsgpu.INIT_TURBULENT, uc=0.0054, amp=0.1, seed=2, nmodes=8)
+            ch.step(2, logs=False)
+            times = {}
+            for _ in range(2):
+                for rows in (1, 2):
+                    for W in (2, 3, 4):
+                        for wpc in (8, 12, 16, 24):
+                            ch.set_step_shape((256 * wpc + W - 1) // W, W, rows)
+                            shape = ch.step_shape(rows=True)
+                            ms, _ = ch.time_steps(args.steps, stats=True)
+                            times.setdefault(shape, []).append(ms / args.steps)
+        finally:
+            ch.close()
+        best = {}
+        for (b, w, r), t in times.items():
+            if r not in best or min(t) < best[r][0]:
+                best[r] = (min(t), w, b)
+        ent = {"grid": name, "nodes": X * Y * Z}
+        for r in (1, 2):
+            tm, w, b = best[r]
+            ent[f"rows{r}"] = {"kernel_ms": round(tm, 4), "waves": w, "blocks": b,
+                               "GBs": round(76 * X * Y * Z / (tm / 1e3) / 1e9)}
+        ent["two_over_one_speed"] = round(best[1][0] / best[2][0], 3)
+        out.append(ent)
+        print(f"# {name}: one row {best[1][0]:.3f} ms, two rows {best[2][0]:.3f} ms", file=sys.stderr, flush=True)
+    return {"metric": "FP16 step kernel, one against two rows per lane", "unit": "ms", "n_gpus": 1,
+            "steps": args.steps, "data": "synthetic", "rows_sweep": out}
+
+
+def run_fp16_drift(args):
+    """Error of the reduced storages against FP64 over the run length, by
+    channel height: the x,y-uniform (laminar) channel at the workloads'
+    lattice viscosity nu = u_tau h / Re_tau with Re_tau = zdv (u_tau 0.0054,
+    so nu = 0.0027 at every zdv), u_b = 15.7 u_tau, on a 4 x 4 x zdv grid,
+    started from the parabola of that u_b.  Constant force (the workloads'
+    forcing) at the parabola's force F = 12 nu u_b / zdv^2: the FP64 state
+    stays on the steady parabola and the storage's error shows as a drift
+    of the bulk velocity u_b; constant flow rate (encalc.c:121): as a drift
+    of the force, a difference of two numbers near u_b (per step, and its
+    mean over the steps since the previous mark).  Per-step changes
+    below half a storage ulp of g are lost, and the force per step falls as
+    1/zdv^2.  Logged at steps 10^2 .. --drift-steps."""
+    import numpy as np
+    zs = [int(z) for z in args.drift_z.split(",")]
+    marks = [m for m in (100, 300, 1000, 3000, 10000, 30000, 100000, 300000, 1000000) if m <= args.drift_steps]
+    precs = (dnsgpu.FP64, dnsgpu.FP32, dnsgpu.FP16)
+    out = []
+    for Z in zs:
+        utau = 0.0054
+        nu = utau * (Z / 2.0) / float(Z)
+        ub = 15.7 * utau
+        F = 12.0 * nu * ub / float(Z * Z)
+        omega = 1.0 / (3.0 * nu + 0.5)
+        for forcing, fname in ((dnsgpu.CONST_F, "constant force"), (dnsgpu.CONST_Q, "constant flow rate")):
+            traj = {}
+            for prec in precs:
+                ch = dnsgpu.Channel(4, 4, Z, Rebs=ub * Z / nu, ub=ub, CFL=0.1, TSTR=0, forcing=forcing,
+                                    force=F if forcing == dnsgpu.CONST_F else 0.0, precision=prec, omega=omega)
+                rows = []
+                try:
+                    ch.init_field(dnsgpu.INIT_POISEUILLE, uc=1.5 * ub, amp=0.0, seed=1)
+                    done = 0
+                    for m in marks:
+                        lg = ch.step(m - done, cntr0=done, logs=True)
+                        done = m
+                        prof = ch.statistics()[0]  # UAVG
+                        # the last step's force and the mean over the steps since the previous mark
+                        rows.append((float(lg["force"][-1]), float(lg["flowrate"][-1]) / (4 * Z), prof,
+                                     float(np.mean(lg["force"]))))
+                finally:
+                    ch.close()
+                traj[prec] = rows
+            for i, m in enumerate(marks):
+                f64, u64, p64, fm64 = traj[dnsgpu.FP64][i]
+                ent = {"Z": Z, "forcing": fname, "steps": m, "force_fp64": f64, "Ub_fp64": u64,
+                       "force_mean_fp64": fm64, "mean_over": [marks[i - 1] if i else 0, m]}
+                for prec in (dnsgpu.FP32, dnsgpu.FP16):
+                    f, u, pr, fm = traj[prec][i]
+                    ent[PNAME[prec]] = {"Ub_rel": abs(u - u64) / abs(u64),
+                                        "force_rel": abs(f - f64) / abs(f64) if f64 else None,
+                                        "force_mean_rel": abs(fm - fm64) / abs(fm64) if fm64 else None,
+                                        "profile_max_rel": float(np.max(np.abs(pr - p64)) / np.max(np.abs(p64)))}
+                out.append(ent)
+            last = out[-1]
+            key = "Ub_rel" if forcing == dnsgpu.CONST_F else "force_mean_rel"
+            print(f"# Z={Z} {fname}: after {marks[-1]} steps {key} fp32 {last['fp32'][key]:.2e} fp16 "
+                  f"{last['fp16'][key]:.2e}, profile fp32 {last['fp32']['profile_max_rel']:.2e} fp16 "
+                  f"{last['fp16']['profile_max_rel']:.2e}", file=sys.stderr, flush=True)
+    return {"metric": "relative error against FP64 of the bulk velocity (constant force), of the force "
+            "(constant flow rate) and of the mean profile", "unit": "1", "n_gpus": 1, "data": "synthetic",
+            "grid": "4 x 4 x zdv, laminar", "drift": out}
+
+
 def roofline_of(r, traffic, kernel):
     return {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": round(r["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": traffic, "kernel": kernel,
@@ -592,6 +705,12 @@ def main():
     ap.add_argument("--dr-runs", default="", help="drag workloads: runs of this process, e.g. fp64:smooth,fp64:textured")
     ap.add_argument("--dr-dir", default="", help="drag workloads: directory keeping each finished run")
     ap.add_argument("--z-sweep", action="store_true", help="step-kernel GB/s against Z (see run_z_sweep)")
+    ap.add_argument("--rows-sweep", action="store_true", help="FP16 one against two rows per lane "
+                    "(see run_rows_sweep)")
+    ap.add_argument("--fp16-drift", action="store_true", help="reduced-storage error against FP64 over steps, "
+                    "by zdv (see run_fp16_drift)")
+    ap.add_argument("--drift-z", default="64,128,192,256,384,512")
+    ap.add_argument("--drift-steps", type=int, default=100000)
     ap.add_argument("--z-list", default="64,96,128,192,384")
     ap.add_argument("--z-prec", default="fp64,fp32,fp16")
     args = ap.parse_args()
@@ -602,7 +721,7 @@ def main():
     # the CPU baselines run before this process touches the GPU (the
     # reference runs as a child process)
     cpu = cpu_ref = cpu_mpi = None
-    if not args.no_cpu and args.workload in WORKLOADS and not args.z_sweep:
+    if not args.no_cpu and args.workload in WORKLOADS and not (args.z_sweep or args.fp16_drift or args.rows_sweep):
         cpu_ref = reference_baseline(args.workload)
         print("# cpu reference (1 rank) done", file=sys.stderr, flush=True)
         cpu_mpi = reference_baseline(args.workload, ranks=REF_MPI_RANKS)
@@ -612,6 +731,12 @@ def main():
     dnsgpu.load()
     if args.z_sweep:
         print(json.dumps(run_z_sweep(args)))
+        return
+    if args.fp16_drift:
+        print(json.dumps(run_fp16_drift(args)))
+        return
+    if args.rows_sweep:
+        print(json.dumps(run_rows_sweep(args)))
         return
     if args.workload == SWEEP:
         print(json.dumps(run_sweep(args)))
@@ -635,13 +760,16 @@ def main():
         # profiled run's own choice
         ent = tr.get(f"{PNAME[p]}_stats") or {}
         shape = f"{runs[p]['step_blocks']}x{64 * runs[p]['step_waves_per_block']}"
+        if runs[p]["step_rows_per_lane"] == 2:
+            shape += "x2rows"
         by = (ent.get("by_shape") or {}).get(shape)
         return (by or ent).get("traffic_bytes_per_launch")
 
     traffic = {p: traffic_of(p) for p in runs}
     tname = {dnsgpu.FP64: "double", dnsgpu.FP32: "float", dnsgpu.FP16: "_Float16"}
     bands = "true" if w.get("riblets") else "false"
-    kname = {p: f"step_kernel<{tname[p]},false,true,{bands}>" for p in runs}
+    kname = {p: f"step_kernel<{tname[p]},false,true,{bands}" + (",2>" if runs[p]["step_rows_per_lane"] == 2 else ">")
+             for p in runs}
     r32 = runs[dnsgpu.FP32]
     r64 = runs.get(dnsgpu.FP64)
     out = {
