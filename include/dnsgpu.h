@@ -77,6 +77,16 @@ enum dnsgpu_precision {
     DNSGPU_FP16 = 2   /* shifted populations in FP16 storage, FP32 collision */
 };
 
+/* FP16 storage against FP64, measured over 10^5 steps of the laminar
+ * channel at the workloads' viscosity (the slowest-changing case): under
+ * constant force the bulk velocity stays within 0.7% and the mean profile
+ * within 1.5% at every zdv up to 512; under constant flow rate the force,
+ * a difference of two numbers near u_b whose laminar value falls as
+ * 1/zdv^2, carries FP16 rounding noise: its mean over 70000 steps is within
+ * 2% of FP64 up to zdv 256, 18% off at 384 and 65% at 512.  dnsrun warns
+ * above this zdv for FP16 with constant flow rate. */
+#define DNSGPU_FP16_CONST_Q_ZMAX 256
+
 enum dnsgpu_forcing {
     DNSGPU_CONST_Q = 0, /* force = rho_mean*ub - jx_mean every step (encalc.c:121) */
     DNSGPU_CONST_F = 1  /* constant body force */

@@ -159,3 +159,20 @@ def test_block_voxeliser_matches_test_replica(lib):
     assert not t.any()
     # footprint area: 2 x 3 cells in each of the 2 x 2 periods
     assert (b == 5).sum() == 2 * 3 * 2 * 2 and set(np.unique(b)) == {0, 5}
+
+
+@pytest.mark.parametrize("zdv,forcing,warned", [(300, "constq", True), (256, "constq", False),
+                                                (300, "constf", False)])
+def test_driver_warns_for_fp16_constant_flow_rate_in_tall_channels(tmp_path, zdv, forcing, warned):
+    """FP16 storage with a constant flow rate above DNSGPU_FP16_CONST_Q_ZMAX
+    prints a warning before the device is touched (here the run then stops:
+    no GPU in this container)."""
+    import subprocess
+    exe = os.path.join(REPO, "dns-solvers_amd", "bin", "dnsrun")
+    (tmp_path / "dns.case").write_text(f"Rebs = 3000.0\nub = 0.08\nxdv = 4\nydv = 4\nzdv = {zdv}\nTSTR = 0\n"
+                                       f"TEND = 1\nCFL = 1.0\nITER = 2\nprecision = fp16\nforcing = {forcing}\n"
+                                       "force = 1e-8\n")
+    env = dict(os.environ, HIP_VISIBLE_DEVICES="")
+    r = subprocess.run([exe, "-s", "dns.case"], cwd=tmp_path, capture_output=True, text=True, timeout=60,
+                       env=env)
+    assert ("warning: FP16 storage with a constant flow rate" in r.stderr) == warned, r.stderr
