@@ -107,11 +107,15 @@ def test_every_handle_call_refuses_a_null_handle():
         "dnsgpu_commit_load": (None,),
         "dnsgpu_save_aos": (None, 0, 1, dp, dp),
         "dnsgpu_get_force": (None, dp),
+        "dnsgpu_set_force": (None, 1e-6),
         "dnsgpu_step": (None, 0, 1, rows),
         "dnsgpu_statistics": (None, dp),
         "dnsgpu_fields": (None, 0, 1, dp, dp),
         "dnsgpu_time_steps": (None, 1, 1, dp, dp),
         "dnsgpu_autotune": (None, 1, ctypes.byref(n), dp),
+        "dnsgpu_step_shape": (None, ctypes.byref(n), ctypes.byref(n), ctypes.byref(n)),
+        "dnsgpu_set_step_shape": (None, 1, 1, 1),
+        "dnsgpu_fields_noalloc": (None, 0, 1, dp, dp),
         "dnsgpu_tavg_reset": (None,),
         "dnsgpu_tavg_sample": (None,),
         "dnsgpu_tavg_get": (None, ctypes.byref(tavg), dp),
@@ -124,6 +128,7 @@ def test_every_handle_call_refuses_a_null_handle():
         "dnsens_time_steps": (None, 1, dp),
         "dnsens_time_kernels": (None, 1, dp, dp),
         "dnsens_grid": (None, ctypes.byref(n), ctypes.byref(n)),
+        "dnsens_bind_thread": (None,),
     }
     for name, args in calls.items():
         assert getattr(L, name)(*args) == EARG, name
