@@ -235,7 +235,6 @@ typedef struct prep_state {
     int ready_head;
     int *ready_req;
     dnsgpu **ready_h;
-    size_t *ready_fp;
     int refused;         /* requests refused by the preparer (status -2) */
     int prepared_all;    /* the preparer has handled every request */
     int stop;            /* the sweep ends: the preparer returns */
@@ -310,7 +309,6 @@ static void *prep_main(void *arg)
         const int slot = p->ready_head + p->nready;
         p->ready_req[slot] = i;
         p->ready_h[slot] = h;
-        p->ready_fp[slot] = held;
         p->nready++;
         pthread_cond_broadcast(&p->cv);
         pthread_mutex_unlock(&p->mu);
@@ -337,12 +335,10 @@ int dnsserve_sweep(const dnsserve_request *req, int n, const dnsserve_config *cf
     memset(&p, 0, sizeof p);
     p.ready_req = malloc(sizeof(int) * (size_t)nn);
     p.ready_h = malloc(sizeof(dnsgpu *) * (size_t)nn);
-    p.ready_fp = malloc(sizeof(size_t) * (size_t)nn);
-    if (!live || !p.ready_req || !p.ready_h || !p.ready_fp) {
+    if (!live || !p.ready_req || !p.ready_h) {
         free(live);
         free(p.ready_req);
         free(p.ready_h);
-        free(p.ready_fp);
         dnsens_destroy(e);
         return DNSGPU_ENOMEM;
     }
@@ -512,7 +508,6 @@ out_nothread:
     free(live);
     free(p.ready_req);
     free(p.ready_h);
-    free(p.ready_fp);
     pthread_mutex_destroy(&p.mu);
     pthread_cond_destroy(&p.cv);
     dnsens_destroy(e);
