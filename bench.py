@@ -83,6 +83,7 @@ def run_precision(wname, prec, steps, warmup):
             ch.init_field(dnsgpu.INIT_TURBULENT, uc=w["utau"], amp=0.1, seed=2, nmodes=8)
         blocks, _ = ch.autotune(trial_steps=5)
         _, waves, rows = ch.step_shape(rows=True)
+        placement = ch.pair_placement()
         ch.step(warmup, cntr0=0, logs=True)
         t0 = time.perf_counter()
         logs = ch.step(steps, cntr0=warmup, logs=True)  # returns after the stream drained
@@ -112,6 +113,7 @@ def run_precision(wname, prec, steps, warmup):
         "step_blocks": blocks,
         "step_waves_per_block": waves,
         "step_rows_per_lane": rows,
+        "pair_placement": {"destinations_tried": placement[0], "step_gain": round(placement[1], 4)},
         "stress_last": float(logs["stress"][-1]),
     }
 
@@ -680,6 +682,44 @@ def run_fp16_drift(args):
             "grid": "4 x 4 x zdv, laminar", "drift": out}
 
 
+def run_placement_sweep(args):
+    """Step-kernel time of fresh handles of one workload with the autotuner's
+    population pair placement on and off, alternating (DNSGPU_PAIR_PLACEMENT),
+    each handle autotuned and timed on its own buffers."""
+    w = WORKLOADS[args.workload]
+    omega, F, ub, nu = case_params(w)
+    X, Y, Z = w["X"], w["Y"], w["Z"]
+    pmap = {"fp64": dnsgpu.FP64, "fp32": dnsgpu.FP32, "fp16": dnsgpu.FP16}
+    esize = {"fp64": 8, "fp32": 4, "fp16": 2}
+    out = {"workload": args.workload, "handles": []}
+    for pname in args.precisions.split(","):
+        for i in range(args.placement_handles):
+            on = i % 2 == 0
+            os.environ["DNSGPU_PAIR_PLACEMENT"] = "1" if on else "0"
+            ch = dnsgpu.Channel(X, Y, Z, Rebs=ub * Z / nu, ub=ub, CFL=0.1, TSTR=0, forcing=dnsgpu.CONST_F,
+                                force=F, precision=pmap[pname], omega=omega)
+            try:
+                if w.get("riblets"):
+                    ch.set_solid_bands(*([riblet_bands(X, Y, *w["riblets"])] * 2))
+                ch.init_field(dnsgpu.INIT_TURBULENT, uc=w["utau"], amp=0.1, seed=2, nmodes=8)
+                t0 = time.perf_counter()
+                ch.autotune(trial_steps=5)
+                t_tune = time.perf_counter() - t0
+                tried, gain = ch.pair_placement()
+                shape = ch.step_shape(rows=True)
+                ch.step(3, logs=False)
+                ms = min(ch.time_steps(10, stats=True)[0], ch.time_steps(10, stats=True)[0]) / 10
+            finally:
+                ch.close()
+            rec = {"precision": pname, "placement": on, "destinations_tried": tried, "step_gain": round(gain, 4),
+                   "shape": list(shape), "autotune_s": round(t_tune, 2), "kernel_ms": round(ms, 3),
+                   "gbs": round(2 * 19 * esize[pname] * X * Y * Z / (ms / 1e3) / 1e9, 1)}
+            out["handles"].append(rec)
+            print("# " + json.dumps(rec), file=sys.stderr, flush=True)
+    os.environ.pop("DNSGPU_PAIR_PLACEMENT", None)
+    return out
+
+
 def roofline_of(r, traffic, kernel):
     return {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": round(r["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": traffic, "kernel": kernel,
@@ -711,6 +751,9 @@ def main():
                     "by zdv (see run_fp16_drift)")
     ap.add_argument("--drift-z", default="64,128,192,256,384,512")
     ap.add_argument("--drift-steps", type=int, default=100000)
+    ap.add_argument("--placement-sweep", action="store_true", help="step time of fresh handles with the "
+                    "population pair placement on and off (see run_placement_sweep)")
+    ap.add_argument("--placement-handles", type=int, default=6)
     ap.add_argument("--z-list", default="64,96,128,192,384")
     ap.add_argument("--z-prec", default="fp64,fp32,fp16")
     args = ap.parse_args()
@@ -721,7 +764,8 @@ def main():
     # the CPU baselines run before this process touches the GPU (the
     # reference runs as a child process)
     cpu = cpu_ref = cpu_mpi = None
-    if not args.no_cpu and args.workload in WORKLOADS and not (args.z_sweep or args.fp16_drift or args.rows_sweep):
+    if not args.no_cpu and args.workload in WORKLOADS and not (args.z_sweep or args.fp16_drift or args.rows_sweep
+                                                               or args.placement_sweep):
         cpu_ref = reference_baseline(args.workload)
         print("# cpu reference (1 rank) done", file=sys.stderr, flush=True)
         cpu_mpi = reference_baseline(args.workload, ranks=REF_MPI_RANKS)
@@ -737,6 +781,9 @@ def main():
         return
     if args.rows_sweep:
         print(json.dumps(run_rows_sweep(args)))
+        return
+    if args.placement_sweep:
+        print(json.dumps(run_placement_sweep(args)))
         return
     if args.workload == SWEEP:
         print(json.dumps(run_sweep(args)))

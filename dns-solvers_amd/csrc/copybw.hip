@@ -6,6 +6,12 @@
 // the practical ceiling of the step kernel's traffic pattern.
 //
 // usage: copybw [GiB=4] [reps=5] [all]
+//        copybw pairs GiB NBUF
+//   pairs: NBUF buffers of GiB allocated one after the other and all held;
+//     prints the copy rate within each buffer (first half into the second)
+//     and of a whole-buffer copy from every buffer into every other one (a
+//     copy between buffers that lie in different halves of device memory
+//     is the faster one)
 //   all: also the copy variants below, a read-only and a write-only stream:
 //     x4       four 16-byte loads in flight per thread
 //     chunks   each block copies one contiguous chunk
@@ -132,8 +138,67 @@ static void launch(int k, int g, const f4* a, f4* b, size_t n, size_t bytes)
     }
 }
 
+// best of three timed grid-stride copies (1024 x 256 threads), GB/s of bytes read + written
+static double pair_rate(const void* a, void* b, size_t bytes, hipEvent_t e0, hipEvent_t e1)
+{
+    float best = 1e30f;
+    for (int r = 0; r < 3; r++) {
+        if (hipEventRecord(e0) != hipSuccess) return -1.0;
+        hipLaunchKernelGGL(copy_f4, dim3(1024), dim3(256), 0, 0, (const f4*)a, (f4*)b, bytes / 16);
+        if (hipGetLastError() != hipSuccess || hipEventRecord(e1) != hipSuccess || hipEventSynchronize(e1) != hipSuccess)
+            return -1.0;
+        float ms;
+        if (hipEventElapsedTime(&ms, e0, e1) != hipSuccess) return -1.0;
+        if (ms < best) best = ms;
+    }
+    return 2.0 * bytes / best / 1e6;
+}
+
+static int pairs(double gib, int nbuf)
+{
+    const size_t bytes = (size_t)(gib * (1ull << 30)) & ~(size_t)65535;
+    if (nbuf < 2 || nbuf > 32) {
+        fprintf(stderr, "pairs: 2 to 32 buffers\n");
+        return 1;
+    }
+    void* buf[32];
+    hipEvent_t e0, e1;
+    CHK(hipEventCreate(&e0));
+    CHK(hipEventCreate(&e1));
+    printf("%d buffers of %zu bytes, allocated in order and held\n", nbuf, bytes);
+    for (int k = 0; k < nbuf; k++) {
+        CHK(hipMalloc(&buf[k], bytes));
+        CHK(hipMemset(buf[k], 0, bytes));
+    }
+    for (int k = 0; k < nbuf; k++) {
+        const double r = pair_rate(buf[k], (char*)buf[k] + bytes / 2, bytes / 2, e0, e1);
+        if (r < 0) return 1;
+        printf("buffer %2d: within (first half -> second half) %7.1f GB/s\n", k, r);
+    }
+    printf("whole-buffer copies, GB/s (row: source, column: destination)\n    ");
+    for (int j = 0; j < nbuf; j++) printf(" %6d", j);
+    printf("\n");
+    for (int i = 0; i < nbuf; i++) {
+        printf("%3d ", i);
+        for (int j = 0; j < nbuf; j++) {
+            if (i == j) {
+                printf(" %6s", "-");
+                continue;
+            }
+            const double r = pair_rate(buf[i], buf[j], bytes, e0, e1);
+            if (r < 0) return 1;
+            printf(" %6.0f", r);
+        }
+        printf("\n");
+        fflush(stdout);
+    }
+    for (int k = 0; k < nbuf; k++) CHK(hipFree(buf[k]));
+    return 0;
+}
+
 int main(int argc, char** argv)
 {
+    if (argc > 3 && !strcmp(argv[1], "pairs")) return pairs(atof(argv[2]), atoi(argv[3]));
     const double gib = argc > 1 ? atof(argv[1]) : 4.0;
     const int reps = argc > 2 ? atoi(argv[2]) : 5;
     const size_t bytes = (size_t)(gib * (1ull << 30)) & ~(size_t)65535;

@@ -588,23 +588,98 @@ static int time_shape(dnsgpu* h, int trial, double* ms_per_launch)
     return 0;
 }
 
+// ------------------------------------------------ population pair placement
+//
+// Measured on MI355X with copy kernels over held allocations (16-34 GiB
+// buffers): device memory falls into two halves, and a copy from one buffer
+// into another runs at 5.45-5.58 TB/s when the two lie in different halves
+// against 5.15-5.28 TB/s within one half; the driver fills one half before
+// the other, so the two population buffers of dnsgpu_create, allocated one
+// after the other, often share a half.  The step reads buf[cur] and writes
+// buf[1 - cur] (then the roles swap), and its time follows the placement of
+// that pair: C4 riblets in FP32 take 13.2 ms on some pairs and 14.2 ms on
+// others, with the same virtual addresses, code and launch shape.  A copy
+// probe does not predict the step well (the step reads three x-planes of
+// the source around the offset it writes), so the placement is timed with
+// the step kernel itself: the destination buffer is replaced by fresh
+// allocations, each held while the next is tried so that the allocator
+// moves on, and the fastest is kept when it beats the original pair.
+namespace {
+constexpr size_t kPairMinBytes = (size_t)2 << 30;      // per buffer: smaller pairs are kept as allocated
+constexpr size_t kPairFreeReserve = (size_t)2 << 30;   // device memory the search leaves free
+constexpr int kPairMaxCandidates = 4;
+constexpr double kPairStop = 0.03;  // a candidate this much faster ends the search
+constexpr double kPairKeep = 0.015; // the fastest candidate replaces the original if this much faster
+}  // namespace
+
+// Pair placement at the current launch shape (state flag already cleared by
+// the autotuner).  The state in buf[cur] is not touched; buf[1 - cur] is
+// overwritten by the trials, as by the autotuner's own.
+static int place_pair(dnsgpu* h, int trial)
+{
+    const size_t bytes = (size_t)h->g.total * h->esize;
+    const char* env = getenv("DNSGPU_PAIR_PLACEMENT");
+    if ((env && !strcmp(env, "0")) || bytes < kPairMinBytes || h->pair_candidates) return 0;
+    auto time_pair = [&](double* t) {
+        double a = 0.0, b = 0.0;
+        int rc = time_shape(h, trial, &a);
+        if (!rc) rc = time_shape(h, trial, &b);
+        *t = std::min(a, b);
+        return rc;
+    };
+    double t0 = 0.0;
+    int rc = time_pair(&t0);
+    if (rc) return rc;
+    void* const orig = h->buf[1 - h->cur];
+    std::vector<void*> held;
+    void* best = nullptr;
+    double tbest = t0;
+    int tried = 0;
+    while (tried < kPairMaxCandidates) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess || free_b < bytes + kPairFreeReserve) break;
+        void* c = nullptr;
+        if (hipMalloc(&c, bytes) != hipSuccess) break;
+        held.push_back(c);
+        tried++;
+        h->buf[1 - h->cur] = c;
+        double t = 0.0;
+        rc = time_pair(&t);
+        h->buf[1 - h->cur] = orig;
+        if (rc) break;
+        if (t < tbest) {
+            tbest = t;
+            best = c;
+        }
+        if (t < (1.0 - kPairStop) * t0) break;
+    }
+    (void)hipGetLastError();  // a refused allocation ends the search, nothing else
+    if (!rc && best && tbest < (1.0 - kPairKeep) * t0) {
+        (void)hipFree(orig);
+        h->buf[1 - h->cur] = best;
+    } else {
+        best = nullptr;
+    }
+    for (void* c : held)
+        if (c != best) (void)hipFree(c);
+    h->pair_candidates = tried;
+    h->pair_gain = best ? t0 / tbest - 1.0 : 0.0;
+    return rc;
+}
+
+extern "C" int dnsgpu_pair_placement(const dnsgpu* h, int* candidates, double* step_gain)
+{
+    if (!h) return set_err(DNSGPU_EARG, "null handle");
+    if (candidates) *candidates = h->pair_candidates;
+    if (step_gain) *step_gain = h->pair_gain;
+    return 0;
+}
+
 extern "C" int dnsgpu_autotune(dnsgpu* h, int trial_steps, int* best_blocks, double* best_ms)
 {
     if (!h || trial_steps < 1) return set_err(DNSGPU_EARG, "bad argument");
     NOT_MEMBER(h);
-    const TuneKey key{h->g.X, h->g.Y, h->g.Z, h->c.precision, h->has_bands};
-    TuneShape cached;
-    if (tune_lookup(key, &cached)) {
-        int rc = set_step_shape(h, cached.waves, cached.blocks, cached.rows);
-        if (rc) return rc;
-        if (best_blocks) *best_blocks = h->step_nblk;
-        if (best_ms) *best_ms = -1.0;
-        return 0;
-    }
-    // time the steady-state kernel over the shapes (waves per block x
-    // waves per CU), then the three fastest again, alternating, to keep a
-    // noisy first round from choosing; the handle's state flag and shape
-    // are restored on every return
+    // the handle's state flag and shape are restored on every return
     struct Restore {
         dnsgpu* h;
         int ps, nblk, w, rows;
@@ -615,6 +690,25 @@ extern "C" int dnsgpu_autotune(dnsgpu* h, int trial_steps, int* best_blocks, dou
         }
     } restore{h, h->poststream, h->step_nblk, h->step_w, h->step_rows};
     h->poststream = 0;
+    const TuneKey key{h->g.X, h->g.Y, h->g.Z, h->c.precision, h->has_bands};
+    TuneShape cached;
+    const bool have = tune_lookup(key, &cached);
+    // the pair first (at the cached or the current shape), so that the
+    // shapes are timed on the buffers the steps will use
+    int rc = have ? set_step_shape(h, cached.waves, cached.blocks, cached.rows) : 0;
+    if (!rc) rc = place_pair(h, trial_steps);
+    if (rc) return rc;
+    if (have) {
+        restore.nblk = h->step_nblk;
+        restore.w = h->step_w;
+        restore.rows = h->step_rows;
+        if (best_blocks) *best_blocks = h->step_nblk;
+        if (best_ms) *best_ms = -1.0;
+        return 0;
+    }
+    // time the steady-state kernel over the shapes (waves per block x
+    // waves per CU), then the three fastest again, alternating, to keep a
+    // noisy first round from choosing
     const int wpcs[] = {2, 3, 4, 6, 8, 12, 16, 20, 24, 32};
     struct Cand {
         TuneShape s;
@@ -626,7 +720,7 @@ extern "C" int dnsgpu_autotune(dnsgpu* h, int trial_steps, int* best_blocks, dou
         for (int W = 2; W <= step_waves(h->g.Z); W++) {
             int last = -1;
             for (int wpc : wpcs) {
-                int rc = set_step_shape(h, W, (h->ncu * wpc + W - 1) / W, R);
+                rc = set_step_shape(h, W, (h->ncu * wpc + W - 1) / W, R);
                 if (rc) return rc;
                 if (h->step_nblk == last || h->step_w != W) continue;
                 last = h->step_nblk;

@@ -72,6 +72,8 @@ struct dnsgpu {
     struct dnsens* owner;  // ensemble the handle is a member of, or null
     int device;            // the device current at dnsgpu_create
     void* scratch;         // device address of that device's static staging array
+    int pair_candidates;   // destination buffers the autotuner's placement tried (0: none)
+    double pair_gain;      // step-time gain of the kept pair over the pair of dnsgpu_create
 };
 
 // Entry points that change or advance a member's state are refused while

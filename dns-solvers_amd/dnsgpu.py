@@ -28,6 +28,7 @@ EXPORTS = (
     "dnsgpu_statistics", "dnsgpu_fields", "dnsgpu_time_steps", "dnsgpu_device_bytes",
     "dnsgpu_autotune", "dnsgpu_tavg_reset", "dnsgpu_tavg_sample", "dnsgpu_tavg_get",
     "dnsgpu_footprint", "dnsgpu_fields_noalloc", "dnsgpu_step_shape", "dnsgpu_set_step_shape", "dnsgpu_set_force",
+    "dnsgpu_pair_placement",
     "dnsens_create", "dnsens_destroy", "dnsens_member_bytes", "dnsens_footprint", "dnsens_add", "dnsens_remove",
     "dnsens_count", "dnsens_bytes_used", "dnsens_budget", "dnsens_bind_thread", "dnsens_step", "dnsens_info", "dnsens_rows",
     "dnsens_autotune", "dnsens_time_steps", "dnsens_time_kernels", "dnsens_grid",
@@ -130,6 +131,7 @@ def load():
         L.dnsgpu_tavg_reset.argtypes = [vp]
         L.dnsgpu_tavg_sample.argtypes = [vp]
         L.dnsgpu_tavg_get.argtypes = [vp, ctypes.POINTER(DnsTavg), dp]
+        L.dnsgpu_pair_placement.argtypes = [vp, ctypes.POINTER(ctypes.c_int), dp]
         L.dnsgpu_device_bytes.argtypes = [vp]
         L.dnsgpu_device_bytes.restype = ctypes.c_size_t
         L.dnsgpu_footprint.argtypes = [ctypes.POINTER(DnsCase)]
@@ -345,6 +347,13 @@ class Channel:
         if profiles:
             out["profiles"] = prof
         return out
+
+    def pair_placement(self):
+        """(destination buffers the autotuner's placement tried (0: none),
+        step-time gain of the kept pair (0: the original pair kept))."""
+        k, g = ctypes.c_int(), ctypes.c_double()
+        _chk(load().dnsgpu_pair_placement(self.h, ctypes.byref(k), ctypes.byref(g)))
+        return k.value, g.value
 
     def device_bytes(self):
         return load().dnsgpu_device_bytes(self.h)

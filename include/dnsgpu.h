@@ -310,6 +310,25 @@ int dnsgpu_set_step_shape(dnsgpu *h, int blocks, int waves_per_block, int rows_p
 /* Device bytes held by the handle. */
 size_t dnsgpu_device_bytes(const dnsgpu *h);
 
+/* Placement of the two population buffers, chosen by dnsgpu_autotune.  On
+ * MI355X device memory falls into two halves, and a stream that reads one
+ * buffer while writing another at the same offsets runs about 6% faster
+ * when the two lie in different halves; the step is such a stream, and the
+ * two buffers of dnsgpu_create, allocated one after the other, often share
+ * a half.  For a case of at least 2 GiB per buffer the autotuner (the
+ * first time it runs on the handle, before it times the launch shapes, at
+ * the cached or current shape) times the step into fresh destination buffers, holding each while the next is
+ * tried (at most 4, as device memory allows, so it briefly holds more than
+ * dnsgpu_footprint), and keeps the fastest when it beats the original pair
+ * by 1.5%; the others are freed.  Only where the bytes live changes, never
+ * a value; the buffer not holding the state is overwritten, as by the
+ * autotuner's trials.  DNSGPU_PAIR_PLACEMENT=0 in the environment switches
+ * it off.  Returns the destination buffers tried (0: none) and the step-time
+ * gain of the kept pair (0 when the original pair was kept); either pointer
+ * may be NULL.  No reference counterpart (the reference's populations live
+ * in host memory). */
+int dnsgpu_pair_placement(const dnsgpu *h, int *candidates, double *step_gain);
+
 /* Device bytes dnsgpu_create would allocate for this case (0 when the case
  * is refused), without allocating: the admission test of a request. */
 size_t dnsgpu_footprint(const dns_case *c);
