@@ -469,6 +469,8 @@ def sweep_measure(prec, burst, sequential=True, kernel=True):
         "aggregate_mlups_busy": round(summ["mlups_busy"], 1),
         "max_resident": summ["max_resident"],
         "peak_device_bytes": summ["peak_bytes"],
+        "pair_arena_s": round(summ["arena_s"], 3),
+        "pair_arena_bytes_per_half": summ["arena_bytes"],
         "status_counts": status,
         "sequential_fp64_s": None if seq is None else round(seq, 3),
         "speedup_vs_sequential_fp64": None if seq is None else round(seq / summ["wall"], 3),
@@ -480,7 +482,10 @@ def sweep_measure(prec, burst, sequential=True, kernel=True):
         "total_steps": int(sum(steps)),
     }
     if kernel:
-        # the dominant kernel: the multi-case step with all 64 solves resident
+        # the dominant kernel: the multi-case step with all 64 solves resident,
+        # their population pairs from the pair arenas as in the sweep
+        esize = 4 if prec == dnsgpu.FP32 else 2
+        dnsgpu.reserve_pair_arenas(sum(r.xdv * r.ydv * r.zdv * 19 * esize for r in reqs))
         hs = [dnsserve.Handle(r, cfg) for r in reqs]
         ens = dnsgpu.Ensemble(precision=prec)
         for h in hs:
@@ -492,6 +497,7 @@ def sweep_measure(prec, burst, sequential=True, kernel=True):
         ens.close()
         for h in hs:
             h.close()
+        dnsgpu.release_pair_arenas()
         achieved = nodes * bpn / (tk / n / 1e3) / 1e9
         tr = measured_traffic(SWEEP)
         traffic = tr[PNAME[prec]]["traffic_bytes_per_launch"] if tr and PNAME[prec] in tr else None

@@ -439,7 +439,15 @@ extern "C" int dnsgpu_create(const dns_case* c, dnsgpu** out)
         return fail(set_err(DNSGPU_EHIP, "stream creation failed"));
     if (hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess)
         return fail(set_err(DNSGPU_EHIP, "event creation failed"));
+    // a small case's population pair from the device's pair arenas when
+    // they are reserved (pairarena.hip), else with the other arrays
+    const bool arena = (size_t)g.total * h->esize < kPairMinBytes &&
+                       pair_arena_alloc(h, (size_t)g.total * h->esize) == 0;
     for (auto& a : handle_allocs(h)) {
+        if (arena && (a.p == &h->buf[0] || a.p == &h->buf[1])) {
+            h->dev_bytes += a.n;
+            continue;
+        }
         if (hipMalloc(a.p, a.n) != hipSuccess)
             return fail(set_err(DNSGPU_ENOMEM, "device allocation of %zu bytes failed", a.n));
         h->dev_bytes += a.n;
@@ -482,6 +490,7 @@ extern "C" void dnsgpu_destroy(dnsgpu* h)
     if (!h) return;
     if (h->owner) (void)dnsens_remove(h->owner, h);
     if (h->stream) (void)hipStreamSynchronize(h->stream);
+    pair_arena_free(h);  // arena ranges go back (buf[0], buf[1] nulled)
     void* ptrs[] = {h->buf[0], h->buf[1], h->mbot,  h->mtop,  h->wword, h->bands, h->bnbr, h->cnbr, h->nf,
                     h->part,   h->gpart,  h->planes, h->shift, h->means, h->tprof, h->slip,
                     h->st,     h->ring,   h->stage};
@@ -605,7 +614,6 @@ static int time_shape(dnsgpu* h, int trial, double* ms_per_launch)
 // allocations, each held while the next is tried so that the allocator
 // moves on, and the fastest is kept when it beats the original pair.
 namespace {
-constexpr size_t kPairMinBytes = (size_t)2 << 30;      // per buffer: smaller pairs are kept as allocated
 constexpr size_t kPairFreeReserve = (size_t)2 << 30;   // device memory the search leaves free
 constexpr int kPairMaxCandidates = 4;
 constexpr double kPairStop = 0.03;  // a candidate this much faster ends the search

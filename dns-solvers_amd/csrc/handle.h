@@ -72,6 +72,7 @@ struct dnsgpu {
     struct dnsens* owner;  // ensemble the handle is a member of, or null
     int device;            // the device current at dnsgpu_create
     void* scratch;         // device address of that device's static staging array
+    int pair_arena;        // 1: buf[0], buf[1] are ranges of the device's pair arenas (pairarena.hip)
     int pair_candidates;   // destination buffers the autotuner's placement tried (0: none)
     double pair_gain;      // step-time gain of the kept pair over the pair of dnsgpu_create
 };
@@ -108,5 +109,14 @@ __attribute__((visibility("hidden"))) int band_pushes(dnsgpu* h);
 // Finisher arguments of the handle (row: log row slot or null).
 __attribute__((visibility("hidden"))) FinalizeArgs fin_args(dnsgpu* h, bool stats, LogRow* row,
                                                                 long long cntr, int from_moments);
+// Pair arenas (pairarena.hip): pair_arena_alloc sets buf[0] and buf[1]
+// (bytes each) from blocks in different halves of device memory and returns
+// 0, or returns nonzero with nothing taken (no arenas reserved on the
+// device, or they are full); pair_arena_free returns them.
+__attribute__((visibility("hidden"))) int pair_arena_alloc(dnsgpu* h, size_t bytes);
+__attribute__((visibility("hidden"))) void pair_arena_free(dnsgpu* h);
+// Population buffers of at least this many bytes each are placed by
+// dnsgpu_autotune (dnsgpu.hip); smaller pairs can come from the arenas.
+constexpr size_t kPairMinBytes = (size_t)2 << 30;
 // One single-case step on the handle's stream (kernels enqueued, no sync).
 __attribute__((visibility("hidden"))) int do_step(dnsgpu* h, bool stats, LogRow* row, long long cntr);

@@ -28,7 +28,7 @@ EXPORTS = (
     "dnsgpu_statistics", "dnsgpu_fields", "dnsgpu_time_steps", "dnsgpu_device_bytes",
     "dnsgpu_autotune", "dnsgpu_tavg_reset", "dnsgpu_tavg_sample", "dnsgpu_tavg_get",
     "dnsgpu_footprint", "dnsgpu_fields_noalloc", "dnsgpu_step_shape", "dnsgpu_set_step_shape", "dnsgpu_set_force",
-    "dnsgpu_pair_placement",
+    "dnsgpu_pair_placement", "dnsgpu_reserve_pair_arenas", "dnsgpu_release_pair_arenas", "dnsgpu_pair_arena_stats",
     "dnsens_create", "dnsens_destroy", "dnsens_member_bytes", "dnsens_footprint", "dnsens_add", "dnsens_remove",
     "dnsens_count", "dnsens_bytes_used", "dnsens_budget", "dnsens_bind_thread", "dnsens_step", "dnsens_info", "dnsens_rows",
     "dnsens_autotune", "dnsens_time_steps", "dnsens_time_kernels", "dnsens_grid",
@@ -132,6 +132,10 @@ def load():
         L.dnsgpu_tavg_sample.argtypes = [vp]
         L.dnsgpu_tavg_get.argtypes = [vp, ctypes.POINTER(DnsTavg), dp]
         L.dnsgpu_pair_placement.argtypes = [vp, ctypes.POINTER(ctypes.c_int), dp]
+        L.dnsgpu_reserve_pair_arenas.argtypes = [ctypes.c_size_t, ctypes.POINTER(ctypes.c_size_t)]
+        L.dnsgpu_release_pair_arenas.argtypes = []
+        L.dnsgpu_pair_arena_stats.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                              ctypes.POINTER(ctypes.c_size_t)]
         L.dnsgpu_device_bytes.argtypes = [vp]
         L.dnsgpu_device_bytes.restype = ctypes.c_size_t
         L.dnsgpu_footprint.argtypes = [ctypes.POINTER(DnsCase)]
@@ -357,6 +361,25 @@ class Channel:
 
     def device_bytes(self):
         return load().dnsgpu_device_bytes(self.h)
+
+
+def reserve_pair_arenas(bytes_per_half):
+    """Pair arenas on the current device (dnsgpu_reserve_pair_arenas);
+    returns the bytes each memory half holds."""
+    got = ctypes.c_size_t()
+    _chk(load().dnsgpu_reserve_pair_arenas(bytes_per_half, ctypes.byref(got)))
+    return got.value
+
+
+def release_pair_arenas():
+    _chk(load().dnsgpu_release_pair_arenas())
+
+
+def pair_arena_stats(device=0):
+    """(blocks of half 0, blocks of half 1, bytes of ranges in use)."""
+    a, b, u = ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()
+    _chk(load().dnsgpu_pair_arena_stats(device, ctypes.byref(a), ctypes.byref(b), ctypes.byref(u)))
+    return a.value, b.value, u.value
 
 
 def footprint(xdv, ydv, zdv, precision=FP64):

@@ -21,7 +21,8 @@ class Config(ctypes.Structure):
     _fields_ = [("precision", ctypes.c_int), ("ub", ctypes.c_double), ("transient", ctypes.c_int),
                 ("window", ctypes.c_int), ("check", ctypes.c_int), ("tol", ctypes.c_double), ("cap", ctypes.c_int),
                 ("budget", ctypes.c_size_t), ("realtime", ctypes.c_int), ("autotune", ctypes.c_int),
-                ("init_modes", ctypes.c_int), ("init_amp", ctypes.c_double), ("verbose", ctypes.c_int)]
+                ("init_modes", ctypes.c_int), ("init_amp", ctypes.c_double), ("verbose", ctypes.c_int),
+                ("pair_arenas", ctypes.c_int)]
 
 
 RESULT_FIELDS = ("arrival", "admitted", "finished", "steps", "nodes", "utau", "Re_tau", "Cf_w", "Cf_p", "Ub",
@@ -33,7 +34,7 @@ class Result(ctypes.Structure):
 
 
 SUMMARY_FIELDS = ("wall", "busy", "node_steps", "mlups", "mlups_busy", "p50", "p99", "mean_ttr", "solves_per_hour",
-                  "peak_bytes")
+                  "peak_bytes", "arena_s", "arena_bytes")
 
 
 class Summary(ctypes.Structure):

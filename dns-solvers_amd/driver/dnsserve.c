@@ -54,6 +54,7 @@ void dnsserve_defaults(dnsserve_config *cfg)
     cfg->autotune = 1;
     cfg->init_modes = 8;
     cfg->init_amp = 0.1;
+    cfg->pair_arenas = 1;
 }
 
 void dnsserve_generate(int n, uint64_t seed, double mean_gap, dnsserve_request *out)
@@ -357,6 +358,18 @@ int dnsserve_sweep(const dnsserve_request *req, int n, const dnsserve_config *cf
     const double t0 = now_s();
     p.t0 = t0;
     double busy = 0.0;
+    if (cfg->pair_arenas) {
+        /* one population buffer of every request in each memory half */
+        const size_t esize = cfg->precision == DNSGPU_FP64 ? 8 : (cfg->precision == DNSGPU_FP16 ? 2 : 4);
+        size_t want = 0;
+        for (int i = 0; i < n; i++) want += (size_t)req[i].xdv * req[i].ydv * req[i].zdv * 19 * esize;
+        if (want > p.budget / 2) want = p.budget / 2;
+        size_t got = 0;
+        rc = dnsgpu_reserve_pair_arenas(want, &got);
+        if (rc) goto out_nothread;
+        sum->arena_bytes = (double)got;
+        sum->arena_s = now_s() - t0;
+    }
     pthread_t th;
     if (pthread_create(&th, NULL, prep_main, &p) != 0) {
         rc = DNSGPU_EARG;
@@ -511,6 +524,7 @@ out_nothread:
     pthread_mutex_destroy(&p.mu);
     pthread_cond_destroy(&p.cv);
     dnsens_destroy(e);
+    if (cfg->pair_arenas) (void)dnsgpu_release_pair_arenas();
     return rc;
 }
 

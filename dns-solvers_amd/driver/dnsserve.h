@@ -56,6 +56,7 @@ typedef struct dnsserve_config {
     int init_modes;    /* Fourier modes of the initial field per periodic direction */
     double init_amp;   /* their amplitude, fraction of the centreline velocity */
     int verbose;       /* 1: a line on stderr per admission and per result */
+    int pair_arenas;   /* 1: reserve pair arenas for the members' population buffers (dnsserve_sweep) */
 } dnsserve_config;
 
 typedef struct dnsserve_result {
@@ -78,6 +79,8 @@ typedef struct dnsserve_summary {
     double mean_ttr;
     double solves_per_hour;
     double peak_bytes;  /* largest admitted footprint sum */
+    double arena_s;     /* seconds spent reserving the pair arenas (inside wall) */
+    double arena_bytes; /* bytes reserved per memory half (0: none) */
     int max_resident;
     int completed;
 } dnsserve_summary;
@@ -108,7 +111,11 @@ int dnsserve_open(const dnsserve_request *r, const dnsserve_config *cfg, dnsgpu 
  * boundary, steps the ensemble and retires.  A request whose footprint
  * exceeds the whole budget is refused (status -2).  Each solve's results
  * do not depend on the mix, on the arrival times or on when it was
- * admitted. */
+ * admitted.  With cfg->pair_arenas the sweep first reserves pair arenas on
+ * the device (dnsgpu_reserve_pair_arenas) for the population buffers of all
+ * n requests (at most half the budget per memory half), so each member's
+ * two buffers lie in different halves of device memory, and releases them
+ * at the end; the reservation's time counts in the wall time. */
 int dnsserve_sweep(const dnsserve_request *req, int n, const dnsserve_config *cfg, dnsserve_result *res,
                    dnsserve_summary *sum);
 

@@ -2,9 +2,10 @@
  * print one JSON object per solve and a summary line.
  *
  *   dnsserve [-n solves] [-s seed] [-g mean_gap_s] [-b] [-p fp32|fp16|fp64]
- *            [-c cap] [-w window] [-t transient] [-k check] [-q seq_precision]
+ *            [-c cap] [-w window] [-t transient] [-k check] [-q seq_precision] [-a]
  *
  *   -b   burst: every request arrives at t = 0 (no arrival times)
+ *   -a   no pair arenas: the members' population buffers allocated plainly
  *   -q   also run the same solves one after another on the single-case
  *        path at this precision (fp64, fp32 or fp16) and report their time
  */
@@ -29,7 +30,7 @@ int main(int argc, char **argv)
     int n = 64, opt, seq = -1;
     unsigned long long seed = 5;
     double gap = 2.0;
-    while ((opt = getopt(argc, argv, "n:s:g:bp:c:w:t:k:q:")) != -1) {
+    while ((opt = getopt(argc, argv, "n:s:g:bp:c:w:t:k:q:a")) != -1) {
         switch (opt) {
         case 'n': n = atoi(optarg); break;
         case 's': seed = strtoull(optarg, NULL, 10); break;
@@ -41,9 +42,10 @@ int main(int argc, char **argv)
         case 't': cfg.transient = atoi(optarg); break;
         case 'k': cfg.check = atoi(optarg); break;
         case 'q': seq = prec_of(optarg); break;
+        case 'a': cfg.pair_arenas = 0; break;
         default:
             fprintf(stderr, "usage: %s [-n solves] [-s seed] [-g gap] [-b] [-p fp32|fp16|fp64] [-c cap] [-w window] "
-                            "[-t transient] [-k check] [-q fp64|fp32]\n", argv[0]);
+                            "[-t transient] [-k check] [-q fp64|fp32] [-a]\n", argv[0]);
             return 2;
         }
     }
@@ -88,9 +90,10 @@ int main(int argc, char **argv)
     }
     printf("{\"summary\": true, \"solves\": %d, \"completed\": %d, \"wall_s\": %.3f, \"busy_s\": %.3f, "
            "\"solves_per_hour\": %.1f, \"p50_s\": %.3f, \"p99_s\": %.3f, \"mean_s\": %.3f, \"mlups\": %.1f, "
-           "\"mlups_busy\": %.1f, \"max_resident\": %d, \"peak_bytes\": %.0f, \"sequential_s\": %.3f}\n",
+           "\"mlups_busy\": %.1f, \"max_resident\": %d, \"peak_bytes\": %.0f, \"pair_arena_s\": %.3f, "
+           "\"pair_arena_bytes_per_half\": %.0f, \"sequential_s\": %.3f}\n",
            n, sum.completed, sum.wall, sum.busy, sum.solves_per_hour, sum.p50, sum.p99, sum.mean_ttr, sum.mlups,
-           sum.mlups_busy, sum.max_resident, sum.peak_bytes, seq_s);
+           sum.mlups_busy, sum.max_resident, sum.peak_bytes, sum.arena_s, sum.arena_bytes, seq_s);
     free(req);
     free(res);
     return 0;

@@ -329,6 +329,25 @@ size_t dnsgpu_device_bytes(const dnsgpu *h);
  * in host memory). */
 int dnsgpu_pair_placement(const dnsgpu *h, int *candidates, double *step_gain);
 
+/* Pair arenas for many small cases (ensemble members).  Reserves on the
+ * current device 8 GiB blocks, each classed by a timed copy probe into one
+ * of the two halves of device memory, until both halves hold
+ * bytes_per_half (as device memory allows; *reserved_per_half receives what
+ * each half holds, may be NULL).  While they exist, dnsgpu_create takes the
+ * two population buffers of a case below 2 GiB per buffer as ranges of a
+ * block of each half (first fit, 2 MiB granules) and falls back to plain
+ * allocations when they are full; a destroyed handle returns its ranges.
+ * Only where the bytes live changes, never a value.  All probing happens
+ * here, before any case steps: reserve before creating the members (the
+ * request loop, dnsserve_sweep, does).  dnsgpu_release_pair_arenas frees
+ * the empty blocks and each other block when its last range returns;
+ * later creations allocate plainly.  No reference counterpart. */
+int dnsgpu_reserve_pair_arenas(size_t bytes_per_half, size_t *reserved_per_half);
+int dnsgpu_release_pair_arenas(void);
+/* Blocks of each half held on `device` and the bytes of their ranges in use
+ * (any pointer may be NULL). */
+int dnsgpu_pair_arena_stats(int device, int *blocks_half0, int *blocks_half1, size_t *bytes_used);
+
 /* Device bytes dnsgpu_create would allocate for this case (0 when the case
  * is refused), without allocating: the admission test of a request. */
 size_t dnsgpu_footprint(const dns_case *c);

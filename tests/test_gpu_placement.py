@@ -9,6 +9,11 @@ populations and logs of an unplaced one at the same launch shape.
   seeded field, against a handle with DNSGPU_PAIR_PLACEMENT=0 at the same
   shape; two x-slabs of both checkpoint halves and every log row bitwise.
 * A small case (below the threshold): the search does not run.
+* Pair arenas (dnsgpu_reserve_pair_arenas): three small members of an
+  ensemble, FP32 and FP16, created while the arenas are reserved (each
+  member's two buffers ranges of blocks in different memory halves) and
+  stepped 37 times, against the same members on plain allocations: states
+  and log rows bitwise; after the release every block is freed.
 """
 import os
 import sys
@@ -73,3 +78,62 @@ def test_small_case_is_not_placed():
             assert ch.pair_placement() == (0, 0.0)
     finally:
         os.environ.pop("DNSGPU_PAIR_PLACEMENT", None)
+
+
+def _members(precision):
+    """Three small members of different grids and textures (seeded
+    Poiseuille fields)."""
+    import fields
+    out = []
+    for (X, Y, Z), tex, seed in (((16, 24, 40), "riblets", 3), ((32, 16, 24), "smooth", 5), ((8, 32, 70), "posts", 7)):
+        ch = dnsgpu.Channel(X, Y, Z, Rebs=60.0, ub=0.05, CFL=0.25, TSTR=5, precision=precision)
+        if tex == "riblets":
+            hb = fields.riblet_bands(X, Y, 8, 4)
+            ch.set_solid_bands(hb, hb)
+        elif tex == "posts":
+            m = fields.posts_mask(X, Y)
+            ch.set_walls(m, m)
+        ch.init_field(dnsgpu.INIT_POISEUILLE, uc=0.06, amp=0.05, seed=seed)
+        out.append(ch)
+    return out
+
+
+def _ensemble_run(precision, arenas):
+    if arenas:
+        got = dnsgpu.reserve_pair_arenas(1 << 30)
+        assert got >= 1 << 30
+        b0, b1, used = dnsgpu.pair_arena_stats(0)
+        assert b0 >= 1 and b1 >= 1 and used == 0
+    try:
+        ms = _members(precision)
+        if arenas:
+            # every member's two buffers are ranges of the arenas
+            assert dnsgpu.pair_arena_stats(0)[2] >= sum(2 * m.X * m.Y * m.Z * 19 * 4 for m in ms)
+        ens = dnsgpu.Ensemble(precision=precision)
+        for m in ms:
+            ens.add(m)
+        ens.step(17)
+        ens.step(20)
+        rows = [ens.rows(m, 20) for m in ms]
+        states = [m.save_aos() for m in ms]
+        ens.close()
+        for m in ms:
+            m.close()
+    finally:
+        if arenas:
+            dnsgpu.release_pair_arenas()
+    return rows, states
+
+
+@pytest.mark.parametrize("precision", [dnsgpu.FP32, dnsgpu.FP16])
+def test_arena_pairs_give_the_same_ensemble_state(precision):
+    print(f"case: pair arenas, three members, precision {precision}, with arenas", flush=True)
+    rows_a, states_a = _ensemble_run(precision, True)
+    assert dnsgpu.pair_arena_stats(0) == (0, 0, 0)  # released: every block freed
+    print(f"case: pair arenas, three members, precision {precision}, plain allocations", flush=True)
+    rows_p, states_p = _ensemble_run(precision, False)
+    for sa, sp in zip(states_a, states_p):
+        assert np.array_equal(sa[0], sp[0]) and np.array_equal(sa[1], sp[1])
+    for ra, rp in zip(rows_a, rows_p):
+        for k in dnsgpu.LOG_FIELDS:
+            assert np.array_equal(ra[k], rp[k]), k
