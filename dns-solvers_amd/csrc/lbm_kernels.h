@@ -343,6 +343,52 @@ __device__ __forceinline__ void gather_wave(const gbyte* const (&sp)[3][3], int 
     }
 }
 
+// gather_wave for a chunk next to a solid band, bounce-back on the read side:
+// a node whose upstream node x - e_a is solid takes its own post-collision
+// opp(a) of the previous step (the population it pushed into the solid) from
+// its own slot.  One load per direction, its address chosen per lane (the
+// upstream slot or the node's own slot of opp(a)), instead of the pull and a
+// masked reload: measured 0.5% faster at 1536x768x384 with riblets (FP32).
+// Upstream wall rows are the wall rule's (band_solid is false there).
+template <typename T>
+__device__ __forceinline__ void gather_wave_read_side(const gbyte* const (&sp)[3][3], int zb, uint32_t zoff,
+                                                      uint32_t zoff_up, uint32_t zoff_dn, bool bot_lane,
+                                                      bool top_lane, uint32_t ww, const uint32_t (&bw)[3][3], int z,
+                                                      int Z, T (&v)[Q])
+{
+    static_for<0, Q>([&](auto ac) {
+        constexpr int a = decltype(ac)::value;
+        const gbyte* up =
+            sp[EX[a] + 1][EY[a] + 1] + a * zb + (EZ[a] == 1 ? zoff_up : (EZ[a] == -1 ? zoff_dn : zoff));
+        if constexpr (a == 0) {
+            v[a] = *(const GLOBAL_AS T*)up;
+        } else {
+            const gbyte* own = sp[1][1] + OPP[a] * zb + zoff;
+            v[a] = *(const GLOBAL_AS T*)(band_solid(bw[1 + EX[a]][1 + EY[a]], z - EZ[a], Z) ? own : up);
+        }
+    });
+    if (bot_lane) {
+        static_for<0, Q>([&](auto ac) {
+            constexpr int a = decltype(ac)::value;
+            if constexpr (EZ[a] == 1) {
+                const bool spec = (ww >> wall_bit<a>()) & 1u;
+                const gbyte* p = spec ? sp[EX[a] + 1][EY[a] + 1] + MIR[a] * zb : sp[1][1] + OPP[a] * zb;
+                v[a] = *(const GLOBAL_AS T*)p;
+            }
+        });
+    }
+    if (top_lane) {
+        static_for<0, Q>([&](auto ac) {
+            constexpr int a = decltype(ac)::value;
+            if constexpr (EZ[a] == -1) {
+                const bool spec = (ww >> wall_bit<a>()) & 1u;
+                const gbyte* p = spec ? sp[EX[a] + 1][EY[a] + 1] + MIR[a] * zb : sp[1][1] + OPP[a] * zb;
+                v[a] = *(const GLOBAL_AS T*)(p + (zb - (int)sizeof(T)));
+            }
+        });
+    }
+}
+
 // The statistics variant keeps the [3][Z] plane-mean shifts in LDS up to
 // Z = 256 and reads them from global memory above: the LDS bounds the blocks
 // resident per CU, and at Z = 384 one more block per CU is worth more than
@@ -412,14 +458,17 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
 {
     const int Z = A.g.Z, nzc = A.g.nzc;
     const int ctop = (Z - 1) / kWave, ltop = (Z - 1) % kWave;
-    // Band surfaces: FP32 storage bounces on the read side (a conditional
-    // reload of the node's own slot, plain stores); FP64 and FP16 on the
-    // write side (a conditional store into the solid neighbour's slot).
-    // Both give the same values.  Measured with riblets (spacing 16, height
-    // 8): the read side is 10% faster in FP32 at 1536x768x384, 5% at
-    // 768x384x192 and 7% in a mixed ensemble; slower in FP64 (3-9%: its
-    // masked reloads wait for the pulls whose registers they overwrite);
-    // in FP16 3.5% faster at 768x384x192 but 3.4% slower in the ensemble.
+    // Band surfaces: FP32 storage bounces on the read side (the node's own
+    // slot loaded in place of the upstream slot, plain stores); FP64 and
+    // FP16 on the write side (a conditional store into the solid
+    // neighbour's slot).  Both give the same values.  Measured with riblets
+    // (spacing 16, height 8): the read side is 10% faster in FP32 at
+    // 1536x768x384, 5% at 768x384x192 and 7% in a mixed ensemble; in one
+    // allocation at 1536x768x384, FP16 4% slower and FP64 1% faster.  A
+    // third rule (a fluid node and its solid neighbour storing into each
+    // other's slot: one store per direction) was 4-10% slower than both:
+    // store instructions whose lanes split between two places cost more
+    // than the extra whole-line stores.
     constexpr bool kReadSide = std::is_same<T, float>::value;
     // Every store of the step is non-temporal: no kernel of the step reads
     // the destination back, so its lines are not kept in the L2, which then
@@ -482,20 +531,13 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
             slow = (maxhb > 0 && z0 - 1 < maxhb) || (maxht > 0 && z1 >= Z - maxht);
             solid_own = band_solid(bw[1][1], z, Z);
         }
-        gather_wave<T, FIRST>(sp, zbc, zoff, zoff_up, zoff_dn, bot_lane, top_lane, ww, v);
         if constexpr (BANDS && !FIRST && kReadSide) {
-            if (slow) {
-                // Halfway bounce-back at band surfaces, on the read side: a
-                // node whose upstream node x - e_a is solid receives its own
-                // post-collision opp(a) of the previous step (the population
-                // it pushed into the solid), from its own slot.  Upstream
-                // wall rows are the wall rule's, above.
-                static_for<1, Q>([&](auto ac) {
-                    constexpr int a = decltype(ac)::value;
-                    if (band_solid(bw[1 + EX[a]][1 + EY[a]], z - EZ[a], Z))
-                        v[a] = *(const GLOBAL_AS T*)(sp[1][1] + OPP[a] * zbc + zoff);
-                });
-            }
+            if (slow)
+                gather_wave_read_side<T>(sp, zbc, zoff, zoff_up, zoff_dn, bot_lane, top_lane, ww, bw, z, Z, v);
+            else
+                gather_wave<T, FIRST>(sp, zbc, zoff, zoff_up, zoff_dn, bot_lane, top_lane, ww, v);
+        } else {
+            gather_wave<T, FIRST>(sp, zbc, zoff, zoff_up, zoff_dn, bot_lane, top_lane, ww, v);
         }
 
         double mass_node, jx_raw;
