@@ -726,6 +726,81 @@ def run_placement_sweep(args):
     return out
 
 
+def _step_variant(name):
+    """(precision, FIRST, STATS, rows per lane) of a step_kernel dispatch name
+    as rocprofv3 writes it (demangled, or mangled for the FP16 instances),
+    or None for any other kernel."""
+    import re
+    m = re.match(r"void lbm::step_kernel<(\w+), (\w+), (\w+), (\w+), (\d)>", name)
+    if m:
+        t, first, stats, _, rows = m.groups()
+        return ({"double": "fp64", "float": "fp32", "_Float16": "fp16"}[t], first == "true", stats == "true",
+                int(rows))
+    m = re.match(r"_ZN3lbmL11step_kernelI(d|f|DF16_)Lb(\d)ELb(\d)ELb(\d)ELi(\d)E", name)
+    if m:
+        t, first, stats, _, rows = m.groups()
+        return {"d": "fp64", "f": "fp32", "DF16_": "fp16"}[t], first == "1", stats == "1", int(rows)
+    return None
+
+
+def run_traffic_summary(args):
+    """HBM bytes per step-kernel dispatch from two rocprofv3 --pmc runs of
+    bench.py (FETCH_SIZE, then WRITE_SIZE; both with the same
+    DNSGPU_TUNE_CACHE, so that they run the same launch shapes) and the same
+    counters over the plain copy of exactly 4 GiB (copybw 4 2): FETCH_SIZE and
+    WRITE_SIZE are scaled by what they read for that copy.  Grouped by
+    kernel variant and launch shape; the shape of a variant's last dispatch
+    is the autotuner's choice (the logged steps).  Prints the traffic record
+    bench.py reads (profiles/traffic_<workload>.json)."""
+    import csv
+    import statistics
+    fetch_csv, write_csv, cal_fetch_csv, cal_write_csv = args.traffic_summary.split(",")
+    copy_kib = 4 * 1024 * 1024
+
+    def cal(path):
+        vals = [float(r["Counter_Value"]) for r in csv.DictReader(open(path)) if "copy_f4" in r["Kernel_Name"]]
+        return copy_kib / statistics.median(vals)
+
+    scale = {"fetch": cal(cal_fetch_csv), "write": cal(cal_write_csv)}
+    w = WORKLOADS[args.workload]
+    nodes = w["X"] * w["Y"] * w["Z"]
+    sums, last = {}, {}
+    for kind, path in (("fetch", fetch_csv), ("write", write_csv)):
+        for r in sorted(csv.DictReader(open(path)), key=lambda r: int(r["Dispatch_Id"])):
+            v = _step_variant(r["Kernel_Name"])
+            if v is None:
+                continue
+            pname, first, stats, rows = v
+            key = f"{pname}_{'first_' if first else ''}{'stats' if stats else 'nostats'}"
+            wg = int(r["Workgroup_Size"])
+            shape = f"{int(r['Grid_Size']) // wg}x{wg}" + ("x2rows" if rows == 2 else "")
+            ent = sums.setdefault(key, {}).setdefault(shape, {"fetch": [], "write": []})
+            ent[kind].append(float(r["Counter_Value"]))
+            last[(key, kind)] = shape  # the logged steps come last
+    out = {"workload": args.workload, "nodes": nodes, "fetch_scale": round(scale["fetch"], 4),
+           "write_scale": round(scale["write"], 4)}
+    for key in sorted(sums):
+        by = {}
+        for shape, ent in sums[key].items():
+            if not ent["fetch"] or not ent["write"]:
+                continue
+            f = statistics.mean(ent["fetch"])
+            wr = statistics.mean(ent["write"])
+            rb = f * 1024 * scale["fetch"]
+            wb = wr * 1024 * scale["write"]
+            by[shape] = {"fetch_kib": round(f), "write_kib": round(wr), "read_B_per_node": round(rb / nodes, 3),
+                         "write_B_per_node": round(wb / nodes, 3), "traffic_bytes_per_launch": round(rb + wb),
+                         "dispatches": min(len(ent["fetch"]), len(ent["write"]))}
+        if not by:
+            continue
+        chosen = last.get((key, "fetch"))
+        if chosen not in by:
+            chosen = max(by, key=lambda k: by[k]["dispatches"])
+        out[key] = dict(chosen_shape=chosen, **{k: v for k, v in by[chosen].items() if k != "dispatches"},
+                        by_shape=by)
+    return out
+
+
 def roofline_of(r, traffic, kernel):
     return {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": round(r["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": traffic, "kernel": kernel,
@@ -760,6 +835,8 @@ def main():
     ap.add_argument("--placement-sweep", action="store_true", help="step time of fresh handles with the "
                     "population pair placement on and off (see run_placement_sweep)")
     ap.add_argument("--placement-handles", type=int, default=6)
+    ap.add_argument("--traffic-summary", default="", help="FETCH.csv,WRITE.csv,COPY_FETCH.csv,COPY_WRITE.csv: "
+                    "the traffic record from rocprofv3 counter runs (see run_traffic_summary)")
     ap.add_argument("--z-list", default="64,96,128,192,384")
     ap.add_argument("--z-prec", default="fp64,fp32,fp16")
     args = ap.parse_args()
@@ -767,6 +844,9 @@ def main():
         print("bench.py: this build runs the whole channel on one GPU; multi-GPU runs are not part of it "
               "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
+    if args.traffic_summary:
+        print(json.dumps(run_traffic_summary(args), indent=1))
+        return
     # the CPU baselines run before this process touches the GPU (the
     # reference runs as a child process)
     cpu = cpu_ref = cpu_mpi = None
@@ -816,9 +896,10 @@ def main():
         if runs[p]["step_rows_per_lane"] == 2:
             shape += "x2rows"
         by = (ent.get("by_shape") or {}).get(shape)
-        return (by or ent).get("traffic_bytes_per_launch")
+        return (by or ent).get("traffic_bytes_per_launch"), shape if by else ent.get("chosen_shape")
 
-    traffic = {p: traffic_of(p) for p in runs}
+    traffic_shape = {p: traffic_of(p) for p in runs}
+    traffic = {p: t for p, (t, _) in traffic_shape.items()}
     tname = {dnsgpu.FP64: "double", dnsgpu.FP32: "float", dnsgpu.FP16: "_Float16"}
     bands = "true" if w.get("riblets") else "false"
     kname = {p: f"step_kernel<{tname[p]},false,true,{bands}" + (",2>" if runs[p]["step_rows_per_lane"] == 2 else ">")
@@ -847,7 +928,8 @@ def main():
                    "per_step": "fused collide-stream + per-step log reductions (encalc/wshearstr)",
                    "parallelism": "single GPU"},
         "roofline": dict(roofline_of(r32, traffic[dnsgpu.FP32], kname[dnsgpu.FP32]),
-                         traffic_source=f"profiles/traffic_{args.workload}.json" if traffic[dnsgpu.FP32] else None),
+                         traffic_source=f"profiles/traffic_{args.workload}.json" if traffic[dnsgpu.FP32] else None,
+                         traffic_shape=traffic_shape[dnsgpu.FP32][1]),
         "speedup_fp32_vs_fp64": round(r32["mlups"] / r64["mlups"], 3) if r64 else None,
         "fp32_shifted": r32,
     }
