@@ -831,9 +831,9 @@ extern "C" int dnsgpu_set_solid_bands(dnsgpu* h, const int16_t* bottom, const in
 {
     if (!h) return set_err(DNSGPU_EARG, "null handle");
     NOT_MEMBER(h);
-    // the FP64 and FP16 steps keep the bounced-back populations in the solid
-    // nodes' slots, which a state advanced without these bands does not have
-    // (one rule for every precision, though FP32 reloads its own slots)
+    // the FP16 step keeps the bounced-back populations in the solid nodes'
+    // slots, which a state advanced without these bands does not have (one
+    // rule for every precision, though FP32 and FP64 load their own slots)
     if (!h->poststream)
         return set_err(DNSGPU_EARG, "solid bands are set before the first step (after create, an init or a load)");
     const int X = h->g.X, Y = h->g.Y, Z = h->g.Z;

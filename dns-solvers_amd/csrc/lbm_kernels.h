@@ -458,18 +458,19 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
 {
     const int Z = A.g.Z, nzc = A.g.nzc;
     const int ctop = (Z - 1) / kWave, ltop = (Z - 1) % kWave;
-    // Band surfaces: FP32 storage bounces on the read side (the node's own
-    // slot loaded in place of the upstream slot, plain stores); FP64 and
+    // Band surfaces: FP32 and FP64 storage bounce on the read side (the
+    // node's own slot loaded in place of the upstream slot, plain stores);
     // FP16 on the write side (a conditional store into the solid
     // neighbour's slot).  Both give the same values.  Measured with riblets
-    // (spacing 16, height 8): the read side is 10% faster in FP32 at
-    // 1536x768x384, 5% at 768x384x192 and 7% in a mixed ensemble; in one
-    // allocation at 1536x768x384, FP16 4% slower and FP64 1% faster.  A
-    // third rule (a fluid node and its solid neighbour storing into each
-    // other's slot: one store per direction) was 4-10% slower than both:
-    // store instructions whose lanes split between two places cost more
-    // than the extra whole-line stores.
-    constexpr bool kReadSide = std::is_same<T, float>::value;
+    // in one allocation each (the two rules switched between timed
+    // launches): the read side is faster in FP32 (10% at 1536x768x384, 5%
+    // at 768x384x192, 7% in a mixed ensemble) and in FP64 (2.9%, 3.1% and
+    // 8.7% at 1536x768x384, 768x384x192 and 384x192x96), slower in FP16
+    // (4%).  A third rule (a fluid node and its solid neighbour storing
+    // into each other's slot: one store per direction) was 4-10% slower
+    // than both: store instructions whose lanes split between two places
+    // cost more than the extra whole-line stores.
+    constexpr bool kReadSide = !std::is_same<T, _Float16>::value;
     // Every store of the step is non-temporal: no kernel of the step reads
     // the destination back, so its lines are not kept in the L2, which then
     // holds the pulled lines a chunk shares with the next one (each e_z = +-1
