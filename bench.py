@@ -801,10 +801,22 @@ def run_traffic_summary(args):
     return out
 
 
+# The read+write ceiling of this device (a plain copy between buffers in
+# different halves of device memory, profiles/copybw_memory_halves.md)
+CEILING_GBS = 5550.0
+
+
 def roofline_of(r, traffic, kernel):
-    return {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": round(r["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": traffic, "kernel": kernel,
-            "kernel_ms": round(r["kernel_ms"], 4), "bytes_per_node": r["bytes_per_node"]}
+    out = {"bound": "hbm", "achieved": round(r["achieved_gbs"], 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+           "frac": round(r["achieved_gbs"] / PEAK_HBM_GBS, 4), "traffic": traffic, "kernel": kernel,
+           "kernel_ms": round(r["kernel_ms"], 4), "bytes_per_node": r["bytes_per_node"],
+           # the same kernel against the device's measured read+write ceiling,
+           # and the rate of the bytes it moves by the HBM counters
+           "frac_of_copy_ceiling": round(r["achieved_gbs"] / CEILING_GBS, 4), "copy_ceiling": CEILING_GBS,
+           "copy_ceiling_source": "profiles/copybw_memory_halves.md"}
+    if traffic:
+        out["traffic_gbs"] = round(traffic / (r["kernel_ms"] / 1e3) / 1e9, 1)
+    return out
 
 
 def main():
