@@ -68,8 +68,12 @@ static int scratch_address(int dev, void** out)
 }
 
 
-// Two rows per lane: FP16 storage and an even number of rows.
-static bool step_rows_ok(int precision, int Z) { return precision == DNSGPU_FP16 && Z >= 2 && Z % 2 == 0; }
+// Two rows per lane: FP16 storage and an even number of rows, one z-piece
+// (up to 512 rows).
+static bool step_rows_ok(int precision, int Z)
+{
+    return precision == DNSGPU_FP16 && Z >= 2 && Z % 2 == 0 && Z <= kWave * kMaxWaves;
+}
 
 // Blocks of the step kernel per CU at W waves per block and `rows` rows per
 // lane (the statistics variant with bands: the largest register set).
@@ -79,7 +83,11 @@ static hipError_t step_occupancy(int precision, int Z, int W, int rows, int* per
     if (rows == 2)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, step_kernel<_Float16, false, true, true, 2>,
                                                             kWave * W, lds);
+    const bool pieces = z_pieces((Z + kWave - 1) / kWave) > 1;
     return with_storage(precision, [&](auto t) {
+        if (pieces)
+            return hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                per, step_kernel<decltype(t), false, true, true, 1, true>, kWave * W, lds);
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, step_kernel<decltype(t), false, true, true>,
                                                             kWave * W, lds);
     });
@@ -155,13 +163,14 @@ FinalizeArgs fin_args(dnsgpu* h, bool stats, LogRow* row, long long cntr, int fr
     return F;
 }
 
-// Partials -> plane sums and globals in h->planes (fixed order).
-static int reduce_partials(dnsgpu* h, bool stats, int nblk)
+// Partials -> plane sums and globals in h->planes (fixed order): nplane
+// partials per plane sum, nglobal per global sum.
+static int reduce_partials(dnsgpu* h, bool stats, int nplane, int nglobal)
 {
     const int Z = h->g.Z;
     if (stats)
-        hipLaunchKernelGGL(reduce_rows_kernel, dim3(kNQ * Z), dim3(256), 0, h->stream, h->part, nblk, h->planes);
-    hipLaunchKernelGGL(reduce_rows_kernel, dim3(kNG), dim3(256), 0, h->stream, h->gpart, nblk,
+        hipLaunchKernelGGL(reduce_rows_kernel, dim3(kNQ * Z), dim3(256), 0, h->stream, h->part, nplane, h->planes);
+    hipLaunchKernelGGL(reduce_rows_kernel, dim3(kNG), dim3(256), 0, h->stream, h->gpart, nglobal,
                        h->planes + (size_t)kNQ * Z);
     HIPCHK(hipGetLastError());
     return 0;
@@ -213,7 +222,13 @@ static void enqueue_step_kernel(dnsgpu* h, bool stats)
             return;
         }
     }
-#define STEP_VARIANT(F, S, B) hipLaunchKernelGGL((step_kernel<T, F, S, B>), grid, block, lds, h->stream, A)
+#define STEP_VARIANT(F, S, B)                                                                      \
+    do {                                                                                           \
+        if (h->g.nzp > 1)                                                                          \
+            hipLaunchKernelGGL((step_kernel<T, F, S, B, 1, true>), grid, block, lds, h->stream, A); \
+        else                                                                                       \
+            hipLaunchKernelGGL((step_kernel<T, F, S, B>), grid, block, lds, h->stream, A);         \
+    } while (0)
     if (h->has_bands) {
         if (h->poststream) {
             if (stats) STEP_VARIANT(true, true, true);
@@ -241,7 +256,7 @@ static int launch_step(dnsgpu* h, bool stats, LogRow* row, long long cntr)
     const bool rs = std::is_same<T, _Float16>::value && h->step_rows == 2 && !h->poststream && h->has_bands;
     enqueue_step_kernel<T>(h, stats);
     HIPCHK(hipGetLastError());
-    int rc = reduce_partials(h, stats, h->step_nblk);
+    int rc = reduce_partials(h, stats, h->step_nblk / h->g.nzp, h->step_nblk);
     if (rc) return rc;
     FinalizeArgs F = fin_args(h, stats, row, cntr, 0);
     hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1024), 0, h->stream, F);
@@ -264,11 +279,11 @@ static int launch_moments(dnsgpu* h)
 {
     StepArgs A = step_args(h, h->buf[h->cur], nullptr, h->nblk);
     if (h->poststream)
-        hipLaunchKernelGGL((moments_kernel<T, true>), dim3(h->nblk), dim3(h->block), 0, h->stream, A);
+        hipLaunchKernelGGL((moments_kernel<T, true>), dim3(h->nblk, h->g.nzp), dim3(h->block), 0, h->stream, A);
     else
-        hipLaunchKernelGGL((moments_kernel<T, false>), dim3(h->nblk), dim3(h->block), 0, h->stream, A);
+        hipLaunchKernelGGL((moments_kernel<T, false>), dim3(h->nblk, h->g.nzp), dim3(h->block), 0, h->stream, A);
     HIPCHK(hipGetLastError());
-    int rc = reduce_partials(h, true, h->nblk);
+    int rc = reduce_partials(h, true, h->nblk, h->nblk * h->g.nzp);
     if (rc) return rc;
     FinalizeArgs F = fin_args(h, true, nullptr, 0, 1);
     hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1024), 0, h->stream, F);
@@ -293,12 +308,21 @@ static int update_wall_words(dnsgpu* h)
     return 0;
 }
 
+// A step grid of about nblk blocks: a positive multiple of the z-piece
+// count (each group of nzp consecutive blocks updates all pieces of its
+// columns).
+static int step_grid(const dnsgpu* h, int nblk)
+{
+    const int P = h->g.nzp;
+    return std::max(P, nblk / P * P);
+}
+
 // Geometry, constants and launch grids of a case (no allocation).
 static int plan_handle(const dns_case* c, dnsgpu* h)
 {
-    if (c->xdv < 2 || c->ydv < 2 || c->zdv < 4 || c->zdv > kWave * kMaxWaves)
+    if (c->xdv < 2 || c->ydv < 2 || c->zdv < 4 || c->zdv > kMaxZ)
         return set_err(DNSGPU_EARG, "grid %dx%dx%d unsupported (xdv,ydv>=2, 4<=zdv<=%d)", c->xdv, c->ydv, c->zdv,
-                       kWave * kMaxWaves);
+                       kMaxZ);
     if ((long long)c->xdv * c->ydv * c->zdv >= (1ll << 31))
         return set_err(DNSGPU_EARG, "grid has more than 2^31 nodes");
     if (c->precision != DNSGPU_FP64 && c->precision != DNSGPU_FP32 && c->precision != DNSGPU_FP16)
@@ -309,11 +333,13 @@ static int plan_handle(const dns_case* c, dnsgpu* h)
     g.Y = c->ydv;
     g.Z = c->zdv;
     g.nzc = (g.Z + kWave - 1) / kWave;
-    g.cpb = std::max(1, 4 / g.nzc);
+    g.nzp = z_pieces(g.nzc);
+    g.cpp = (g.nzc + g.nzp - 1) / g.nzp;
+    g.cpb = std::max(1, 4 / g.cpp);
     g.ncols = g.X * g.Y;
     g.ngroups = (g.ncols + g.cpb - 1) / g.cpb;
     g.total = (long long)g.ncols * Q * g.Z;
-    h->block = kWave * g.nzc * g.cpb;
+    h->block = kWave * g.cpp * g.cpb;
     h->step_w = step_waves(g.Z);
     const char* swe = getenv("DNSGPU_STEP_WAVES");
     if (swe) h->step_w = std::max(1, std::min(atoi(swe), h->step_w));
@@ -368,9 +394,10 @@ static int plan_handle(const dns_case* c, dnsgpu* h)
     const char* sbe = getenv("DNSGPU_STEP_BLOCKS");
     if (sbe) sb = atoi(sbe);
     // the partial-sum buffers hold one row per block for any block shape
-    // the autotuner may choose (down to one wave per block)
-    h->step_nblk_max = std::max(1, std::min(g.ncols, 64 * ncu));
-    h->step_nblk = std::max(1, std::min(sb, h->step_nblk_max));
+    // the autotuner may choose (down to one wave per block); the step grid
+    // is a multiple of the z-piece count
+    h->step_nblk_max = std::max(g.nzp, std::min(g.ncols * g.nzp, 64 * ncu) / g.nzp * g.nzp);
+    h->step_nblk = step_grid(h, std::min(sb, h->step_nblk_max));
     return 0;
 }
 
@@ -385,7 +412,9 @@ static std::vector<DevAlloc> handle_allocs(dnsgpu* h)
     const Geom& g = h->g;
     const size_t popbytes = (size_t)g.total * h->esize;
     const size_t xy = (size_t)g.X * g.Y;
-    const size_t np = (size_t)std::max(h->nblk, h->step_nblk_max);
+    // partials: nblk per plane of the chunk-mapped kernels and nblk * nzp
+    // per global sum; the step grid's (up to step_nblk_max)
+    const size_t np = (size_t)std::max(h->nblk * g.nzp, h->step_nblk_max);
     return {
         {&h->buf[0], popbytes},
         {&h->buf[1], popbytes},
@@ -575,8 +604,8 @@ static int set_step_shape(dnsgpu* h, int W, int nblk, int rows)
     h->step_w = W;
     h->step_rows = rows;
     h->step_resident = std::max(1, per) * h->ncu;
-    const int cols = std::max(1, (h->g.ncols + W - 1) / W);
-    h->step_nblk = std::max(1, std::min(nblk, std::min(std::min(h->step_nblk_max, h->step_resident), cols)));
+    const int cols = std::max(1, (h->g.ncols + W - 1) / W) * h->g.nzp;
+    h->step_nblk = step_grid(h, std::min(nblk, std::min(std::min(h->step_nblk_max, h->step_resident), cols)));
     return 0;
 }
 
@@ -772,7 +801,8 @@ extern "C" int dnsgpu_set_step_shape(dnsgpu* h, int blocks, int waves_per_block,
         return set_err(DNSGPU_EARG, "launch shape %d blocks x %d waves (1..%d waves at zdv %d)", blocks,
                        waves_per_block, step_waves(h->g.Z), h->g.Z);
     if (rows_per_lane != 1 && !(rows_per_lane == 2 && step_rows_ok(h->c.precision, h->g.Z)))
-        return set_err(DNSGPU_EARG, "%d rows per lane (2 needs FP16 storage and an even zdv)", rows_per_lane);
+        return set_err(DNSGPU_EARG, "%d rows per lane (2 needs FP16 storage and an even zdv <= %d)", rows_per_lane,
+                       kWave * kMaxWaves);
     return set_step_shape(h, waves_per_block, blocks, rows_per_lane);
 }
 
@@ -1225,7 +1255,7 @@ static int launch_stats_pass(dnsgpu* h, int pass)
     A.means = h->means;
     A.part = h->part;
     A.nblk = h->nblk;
-    hipLaunchKernelGGL(stats_kernel<T>, dim3(h->nblk), dim3(h->block), 0, h->stream, A);
+    hipLaunchKernelGGL(stats_kernel<T>, dim3(h->nblk, h->g.nzp), dim3(h->block), 0, h->stream, A);
     HIPCHK(hipGetLastError());
     hipLaunchKernelGGL(reduce_rows_kernel, dim3(5 * h->g.Z), dim3(256), 0, h->stream, h->part, h->nblk, h->planes);
     HIPCHK(hipGetLastError());
@@ -1483,7 +1513,7 @@ extern "C" int dnsgpu_time_steps(dnsgpu* h, int nsteps, int with_stats, double* 
         with_storage(h->c.precision, [&](auto t) { enqueue_step_kernel<decltype(t)>(h, st); });
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(ev[2 * s + 1], h->stream));
-        int rc = reduce_partials(h, st, h->step_nblk);
+        int rc = reduce_partials(h, st, h->step_nblk / h->g.nzp, h->step_nblk);
         if (rc) return rc;
         FinalizeArgs F = fin_args(h, st, nullptr, s, 0);
         hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1024), 0, h->stream, F);

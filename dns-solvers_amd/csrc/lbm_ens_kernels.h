@@ -111,7 +111,8 @@ static __global__ __launch_bounds__(kWave* kEnsWaves) void ens_step_kernel(const
         // one code path for every member: a member without solid bands has
         // all-zero band records, so no chunk takes the band branch
         for (int k = 0; k < tl.ncol; k++)
-            step_column<T, false, true, true>(A, tl.col0 + k * tl.stride, lane, G, SHL ? shl : A.shift, mine, acc);
+            step_column<T, false, true, true>(A, tl.col0 + k * tl.stride, lane, G, SHL ? shl : A.shift, mine, acc,
+                                              0, A.g.nzc, Z, 0);
         wave_lds_sync();
         double* rec = A.part + (long long)tl.lt * ens_record(Z);
         for (int i = lane; i < kNQ * Z; i += kWave) rec[i] = mine[i];

@@ -194,11 +194,11 @@ static __global__ __launch_bounds__(kWave* kMaxWaves) void stats_kernel(StatArgs
     __shared__ double red[kMaxWaves][5][kWave];
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int nzc = A.g.nzc;
-    const int zc = wave % nzc;
-    const int slot = wave / nzc;
+    const int cpp = A.g.cpp;
+    const int zcl = wave % cpp;  // chunk within the block's z-piece (blockIdx.y)
+    const int slot = wave / cpp;
     const int X = A.g.X, Y = A.g.Y, Z = A.g.Z;
-    const int z = zc * kWave + lane;
+    const int z = (blockIdx.y * cpp + zcl) * kWave + lane;
     const bool zvalid = z < Z;
     double acc[5] = {0., 0., 0., 0., 0.};
     double mu = 0., mv = 0., mw = 0., md = 0.;
@@ -234,11 +234,11 @@ static __global__ __launch_bounds__(kWave* kMaxWaves) void stats_kernel(StatArgs
 #pragma unroll
     for (int q = 0; q < 5; q++) red[wave][q][lane] = acc[q];
     __syncthreads();
-    if (wave < nzc && z < Z) {
+    if (wave < cpp && z < Z) {
 #pragma unroll
         for (int q = 0; q < 5; q++) {
             double s = 0.0;
-            for (int sl = 0; sl < A.g.cpb; sl++) s += red[sl * nzc + zc][q][lane];
+            for (int sl = 0; sl < A.g.cpb; sl++) s += red[sl * cpp + zcl][q][lane];
             A.part[((long long)q * Z + z) * A.nblk + blockIdx.x] = s;
         }
     }

@@ -43,10 +43,20 @@ constexpr int kMaxWaves = 8;  // waves per block
 constexpr int kNQ = 4;        // per-plane accumulators: su sv sw sdrho
 constexpr int kNG = 4;        // global accumulators: jx, mass-1, budget momentum, sq (K)
 constexpr int kNAcc = kNQ + kNG;
+// Tallest column: the finisher owns one plane per thread of its block.
+constexpr int kMaxZ = 1024;
+
+// A column taller than kMaxWaves chunks (512 rows) is split into z-pieces
+// of at most kMaxWaves chunks: the step kernel gives each piece to its own
+// group of blocks (each wave's plane sums then cover one piece), and the
+// chunk-mapped kernels to the y index of their grid.
+__host__ __device__ inline int z_pieces(int nzc) { return (nzc + kMaxWaves - 1) / kMaxWaves; }
 
 struct Geom {
     int X, Y, Z;
     int nzc;          // 64-row chunks per column
+    int nzp;          // z-pieces per column (1 up to 512 rows)
+    int cpp;          // chunks per z-piece: ceil(nzc / nzp)
     int cpb;          // columns per block iteration (chunk-mapped kernels)
     int ncols;        // X*Y
     int ngroups;      // ceil(ncols / cpb)
@@ -396,13 +406,22 @@ __device__ __forceinline__ void gather_wave_read_side(const gbyte* const (&sp)[3
 // copy is 3-5% faster).
 __host__ __device__ inline bool step_shifts_in_lds(int Z) { return Z <= 256; }
 
+// Planes of one wave's plane sums in the step kernel: the column, or above
+// 512 rows one z-piece of it (whole chunks).
+__host__ __device__ inline int step_plane_rows(int Z)
+{
+    const int nzc = (Z + kWave - 1) / kWave, P = z_pieces(nzc);
+    return P == 1 ? Z : (nzc + P - 1) / P * kWave;
+}
+
 // Dynamic LDS of the step kernel: [W][kNG] per-wave global sums, then with
-// STATS [W][kNQ][Z] per-wave plane sums and (see above) the [3][Z] shifts.
-// At Z = 192, 4 waves: 29 KB, five blocks per CU.
+// STATS [W][kNQ][PZ] per-wave plane sums (PZ = step_plane_rows) and (see
+// above) the [3][Z] shifts.  At Z = 192, 4 waves: 29 KB, five blocks per CU.
 __host__ __device__ inline size_t step_lds_bytes(int Z, int W, bool stats)
 {
-    return sizeof(double) *
-           ((size_t)W * kNG + (stats ? (size_t)W * kNQ * Z + (step_shifts_in_lds(Z) ? 3 * (size_t)Z : 0) : 0));
+    return sizeof(double) * ((size_t)W * kNG + (stats ? (size_t)W * kNQ * step_plane_rows(Z) +
+                                                             (step_shifts_in_lds(Z) ? 3 * (size_t)Z : 0)
+                                                       : 0));
 }
 
 // Waves per block of the step kernel for a given Z: at most 4, and the
@@ -452,11 +471,13 @@ struct StepAcc {
 // term, whose x-moment is exactly G, and streaming is a permutation except
 // that a halfway bounce-back reverses e_x:
 //     sum f_next = sum f,  sum e_x f_next = sum (j_x + G) - 2 sum_{BB} e_x g
+//   Chunks c0 <= ch < c1 of the column (the whole column, or one z-piece);
+//   mine holds planes zbase .. zbase + pz - 1 as [kNQ][pz].
 template <typename T, bool FIRST, bool STATS, bool BANDS>
 __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane, double G, const double* shl,
-                                            double* mine, StepAcc& acc)
+                                            double* mine, StepAcc& acc, int c0, int c1, int pz, int zbase)
 {
-    const int Z = A.g.Z, nzc = A.g.nzc;
+    const int Z = A.g.Z;
     const int ctop = (Z - 1) / kWave, ltop = (Z - 1) % kWave;
     // Band surfaces: FP32 and FP64 storage bounce on the read side (the
     // node's own slot loaded in place of the upstream slot, plain stores);
@@ -509,9 +530,10 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
         maxhb = (int)(nb.w[9] & 0xFFFFu);
         maxht = (int)(nb.w[9] >> 16);
     }
-    for (int ch = 0; ch < nzc; ch++) {
+    for (int ch = c0; ch < c1; ch++) {
         const int z = ch * kWave + lane;
         if (z >= Z) break;
+        const int zm = z - zbase;  // plane slot in mine
         const bool bot_lane = ch == 0 && lane == 0;
         const bool top_lane = ch == ctop && lane == ltop;
         const uint32_t zoff = (uint32_t)z * (uint32_t)sizeof(T);
@@ -552,10 +574,10 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
             if constexpr (STATS) {
                 if (!solid_own) {
                     const double du = u0 - shl[z], dv = u1 - shl[Z + z], dw = u2 - shl[2 * Z + z];
-                    mine[0 * Z + z] += du;
-                    mine[1 * Z + z] += dv;
-                    mine[2 * Z + z] += dw;
-                    mine[3 * Z + z] += mass_node;
+                    mine[0 * pz + zm] += du;
+                    mine[1 * pz + zm] += dv;
+                    mine[2 * pz + zm] += dw;
+                    mine[3 * pz + zm] += mass_node;
                     acc.sq += (du * du + dv * dv) + dw * dw;
                 }
             }
@@ -568,10 +590,10 @@ __device__ __forceinline__ void step_column(const StepArgs& A, int col, int lane
                 if (!solid_own) {
                     const double du = (double)u0 - shl[z], dv = (double)u1 - shl[Z + z],
                                  dw = (double)u2 - shl[2 * Z + z];
-                    mine[0 * Z + z] += du;
-                    mine[1 * Z + z] += dv;
-                    mine[2 * Z + z] += dw;
-                    mine[3 * Z + z] += mass_node;
+                    mine[0 * pz + zm] += du;
+                    mine[1 * pz + zm] += dv;
+                    mine[2 * pz + zm] += dw;
+                    mine[3 * pz + zm] += mass_node;
                     acc.sq += (du * du + dv * dv) + dw * dw;
                 }
             }
@@ -877,12 +899,14 @@ static __global__ __launch_bounds__(256) void band_push_kernel(T* __restrict__ s
 
 // Deterministic reductions of one block: plane sums over the block's waves
 // in wave order, global sums by a fixed butterfly over each wave's lanes and
-// then over the waves in order; written as partial number `slot` of
-// [kNQ][Z][nslot] and [kNG][nslot].
+// then over the waves in order.  The waves' plane sums cover planes zbase ..
+// zbase + pz - 1 ([W][kNQ][pz]); they are written as partial number pslot
+// of [kNQ][Z][npslot], the global sums as partial number gslot of
+// [kNG][ngslot].
 template <bool STATS>
 __device__ __forceinline__ void step_block_reduce(double* red, const double* planes, const StepAcc& acc, int Z,
-                                                  int W, int wave, int lane, double* part, double* gpart, int nslot,
-                                                  int slot)
+                                                  int pz, int zbase, int W, int wave, int lane, double* part,
+                                                  double* gpart, int npslot, int pslot, int ngslot, int gslot)
 {
     // each wave's global sums over its lanes (fixed butterfly), then over the
     // block's waves in wave order
@@ -898,11 +922,12 @@ __device__ __forceinline__ void step_block_reduce(double* red, const double* pla
     }
     __syncthreads();
     if constexpr (STATS) {
-        for (int i = threadIdx.x; i < kNQ * Z; i += blockDim.x) {
+        for (int i = threadIdx.x; i < kNQ * pz; i += blockDim.x) {
+            const int q = i / pz, z = zbase + (i - q * pz);
+            if (z >= Z) continue;
             double s = 0.0;
-            for (int w = 0; w < W; w++) s += planes[(size_t)w * kNQ * Z + i];
-            const int q = i / Z, z = i - q * Z;
-            part[((long long)q * Z + z) * nslot + slot] = s;
+            for (int w = 0; w < W; w++) s += planes[(size_t)w * kNQ * pz + i];
+            part[((long long)q * Z + z) * npslot + pslot] = s;
         }
     }
     if (threadIdx.x < kNG) {
@@ -910,7 +935,7 @@ __device__ __forceinline__ void step_block_reduce(double* red, const double* pla
         if (STATS || (q != 0 && q != 3)) {
             double s = 0.0;
             for (int w = 0; w < W; w++) s += red[w * kNG + q];
-            gpart[(long long)q * nslot + slot] = s;
+            gpart[(long long)q * ngslot + gslot] = s;
         }
     }
 }
@@ -919,33 +944,45 @@ __device__ __forceinline__ void step_block_reduce(double* red, const double* pla
 // the grid; one set of partials per block.
 template <typename T, bool FIRST, bool STATS, bool BANDS, int R>
 __device__ __forceinline__ void step_column_of(const StepArgs& A, int col, int lane, double G, const double* shl,
-                                               double* mine, StepAcc& acc)
+                                               double* mine, StepAcc& acc, int c0, int c1, int pz, int zbase)
 {
     if constexpr (R == 2) {
         static_assert(std::is_same<T, _Float16>::value && !FIRST, "two rows per lane: FP16 pull steps only");
-        step_column_r2<STATS, BANDS>(A, col, lane, G, shl, mine, acc);
+        step_column_r2<STATS, BANDS>(A, col, lane, G, shl, mine, acc);  // one piece (Z <= 512)
     } else {
-        step_column<T, FIRST, STATS, BANDS>(A, col, lane, G, shl, mine, acc);
+        step_column<T, FIRST, STATS, BANDS>(A, col, lane, G, shl, mine, acc, c0, c1, pz, zbase);
     }
 }
 
 // R: rows per lane (1, or 2 for FP16 storage with an even Z).
-template <typename T, bool FIRST, bool STATS, bool BANDS, int R = 1>
+// PIECES: columns above 512 rows, split into A.g.nzp z-pieces (gridDim.x a
+// multiple of nzp: block b takes piece b % nzp of the columns of its group
+// b / nzp); without it the whole column, with compile-time piece bounds.
+template <typename T, bool FIRST, bool STATS, bool BANDS, int R = 1, bool PIECES = false>
 static __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs A)
 {
+    static_assert(!PIECES || R == 1, "two rows per lane: one z-piece");
     extern __shared__ double lds[];  // see step_lds_bytes
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
     const int W = blockDim.x / kWave;
     const int Z = A.g.Z;
+    const int P = PIECES ? A.g.nzp : 1;
+    const int piece = PIECES ? (int)blockIdx.x % P : 0;
+    const int bgrp = PIECES ? (int)blockIdx.x / P : (int)blockIdx.x;
+    const int ngrp = PIECES ? (int)gridDim.x / P : (int)gridDim.x;
+    const int c0 = PIECES ? piece * A.g.cpp : 0;
+    const int c1 = PIECES ? min(A.g.nzc, c0 + A.g.cpp) : A.g.nzc;
+    const int pz = PIECES ? A.g.cpp * kWave : Z;  // planes of each wave's sums
+    const int zbase = PIECES ? c0 * kWave : 0;
     double* red = lds;  // [W][kNG]
     double* planes = lds + (size_t)W * kNG;
-    double* mine = planes + (size_t)wave * kNQ * Z;
-    double* shl = planes + (size_t)W * kNQ * Z;
+    double* mine = planes + (size_t)wave * kNQ * pz;
+    double* shl = planes + (size_t)W * kNQ * pz;
     const bool shl_lds = step_shifts_in_lds(Z);
 
     if constexpr (STATS) {
-        for (int i = lane; i < kNQ * Z; i += kWave) mine[i] = 0.0;
+        for (int i = lane; i < kNQ * pz; i += kWave) mine[i] = 0.0;
         if (shl_lds)
             for (int i = threadIdx.x; i < 3 * Z; i += blockDim.x) shl[i] = A.shift[i];
         __syncthreads();
@@ -953,16 +990,17 @@ static __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs 
     StepAcc acc;
     const double G = *A.G;
     if (!STATS || shl_lds) {
-        for (int it = blockIdx.x * W + wave; it < A.g.ncols; it += gridDim.x * W)
+        for (int it = bgrp * W + wave; it < A.g.ncols; it += ngrp * W)
             step_column_of<T, FIRST, STATS, BANDS, R>(A, __builtin_amdgcn_readfirstlane(it), lane, G, shl, mine,
-                                                      acc);
+                                                      acc, c0, c1, pz, zbase);
     } else {
         // the same loop with the shifts read from global memory
-        for (int it = blockIdx.x * W + wave; it < A.g.ncols; it += gridDim.x * W)
+        for (int it = bgrp * W + wave; it < A.g.ncols; it += ngrp * W)
             step_column_of<T, FIRST, STATS, BANDS, R>(A, __builtin_amdgcn_readfirstlane(it), lane, G, A.shift,
-                                                      mine, acc);
+                                                      mine, acc, c0, c1, pz, zbase);
     }
-    step_block_reduce<STATS>(red, planes, acc, Z, W, wave, lane, A.part, A.gpart, A.nblk, blockIdx.x);
+    step_block_reduce<STATS>(red, planes, acc, Z, pz, zbase, W, wave, lane, A.part, A.gpart, ngrp, bgrp,
+                             (int)gridDim.x, (int)blockIdx.x);
 }
 
 // Per-column wall words from the wall masks (the pull table of the push
@@ -1030,32 +1068,37 @@ static __global__ __launch_bounds__(256) void band_nbr_kernel(const uint32_t* __
 }
 
 // Per-lane accumulators of the chunk-mapped kernels (one wave per
-// (column, 64-row chunk)): per-plane sums over the cpb column slots in
-// fixed order, global sums in a fixed order, written as partials.
+// (column, 64-row chunk) of the block's z-piece, blockIdx.y): per-plane
+// sums over the cpb column slots in fixed order, written as partial
+// blockIdx.x of [kNQ][Z][gridDim.x]; global sums in a fixed order, written
+// as partial blockIdx.y * gridDim.x + blockIdx.x of [kNG][gridDim.x *
+// gridDim.y].  zcl: the wave's chunk within the piece.
 __device__ __forceinline__ void block_reduce_write(const StepArgs& A, double (&acc)[kNAcc], int wave, int lane,
-                                                   int zc, int z)
+                                                   int zcl, int z)
 {
     __shared__ double red[kMaxWaves][kNAcc][kWave];
-    const int nw = A.g.nzc * A.g.cpb;
+    const int cpp = A.g.cpp;
+    const int nw = cpp * A.g.cpb;
 #pragma unroll
     for (int q = 0; q < kNAcc; q++) red[wave][q][lane] = acc[q];
     __syncthreads();
-    if (wave < A.g.nzc && z < A.g.Z) {
+    if (wave < cpp && z < A.g.Z) {
 #pragma unroll
         for (int q = 0; q < kNQ; q++) {
             double s = 0.0;
-            for (int sl = 0; sl < A.g.cpb; sl++) s += red[sl * A.g.nzc + zc][q][lane];
-            A.part[((long long)q * A.g.Z + z) * A.nblk + blockIdx.x] = s;
+            for (int sl = 0; sl < A.g.cpb; sl++) s += red[sl * cpp + zcl][q][lane];
+            A.part[((long long)q * A.g.Z + z) * gridDim.x + blockIdx.x] = s;
         }
     }
     if (wave == 0) {
+        const long long ng = (long long)gridDim.x * gridDim.y, gs = (long long)blockIdx.y * gridDim.x + blockIdx.x;
 #pragma unroll
         for (int q = kNQ; q < kNAcc; q++) {
             double s = 0.0;
             for (int w = 0; w < nw; w++) s += red[w][q][lane];
 #pragma unroll
             for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
-            if (lane == 0) A.gpart[(long long)(q - kNQ) * A.nblk + blockIdx.x] = s;
+            if (lane == 0) A.gpart[(long long)(q - kNQ) * ng + gs] = s;
         }
     }
 }
@@ -1068,11 +1111,10 @@ static __global__ __launch_bounds__(kWave* kMaxWaves) void moments_kernel(StepAr
 {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    const int nzc = A.g.nzc;
-    const int zc = wave % nzc;
-    const int slot = wave / nzc;
+    const int zcl = wave % A.g.cpp;  // chunk within the block's z-piece
+    const int slot = wave / A.g.cpp;
     const int X = A.g.X, Y = A.g.Y, Z = A.g.Z;
-    const int z = zc * kWave + lane;
+    const int z = (blockIdx.y * A.g.cpp + zcl) * kWave + lane;
     const bool zvalid = z < Z;
     const T* __restrict__ src = (const T*)A.src;
     double acc[kNAcc];
@@ -1117,7 +1159,7 @@ static __global__ __launch_bounds__(kWave* kMaxWaves) void moments_kernel(StepAr
         acc[kNQ + 2] += ux;  // momentum of the state itself
         acc[kNQ + 3] += (du * du + dv * dv) + dw * dw;
     }
-    block_reduce_write(A, acc, wave, lane, zc, z);
+    block_reduce_write(A, acc, wave, lane, zcl, z);
 }
 
 // Fixed-order reduction of the partials: block b sums row b of a

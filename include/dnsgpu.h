@@ -168,7 +168,7 @@ typedef struct dnsgpu dnsgpu;
 /* Allocate the device state for a case.  Replaces the calloc block of
  * dns-const-q-xycut.c:239-356 (no halo buffers: the periodic x/y wrap is
  * done inside the kernel).  DNSGPU_EARG, with *out = NULL, for a grid
- * outside xdv, ydv >= 2, 4 <= zdv <= 512, fewer than 2^31 nodes, or for
+ * outside xdv, ydv >= 2, 4 <= zdv <= 1024, fewer than 2^31 nodes, or for
  * parameters the reference would run into NaN with: a relaxation rate
  * (given, or derived from Rebs and ub) outside (0, 2), CFL <= 0, a
  * non-finite ub or force, an unknown forcing or precision. */
@@ -302,9 +302,11 @@ int dnsgpu_autotune(dnsgpu *h, int trial_steps, int *best_blocks, double *best_m
  * counterpart. */
 int dnsgpu_step_shape(const dnsgpu *h, int *blocks, int *waves_per_block, int *rows_per_lane);
 /* Sets the launch shape by hand (blocks are capped at what is resident at
- * once); rows_per_lane 2 needs FP16 storage and an even zdv.  The state
- * and the results do not depend on the shape beyond the summation order of
- * the logged sums.  No reference counterpart. */
+ * once and, above 512 rows, rounded down to a multiple of the column's
+ * z-pieces of at most 512 rows); rows_per_lane 2 needs FP16 storage and an
+ * even zdv up to 512.  The state and the results do not depend on the
+ * shape beyond the summation order of the logged sums.  No reference
+ * counterpart. */
 int dnsgpu_set_step_shape(dnsgpu *h, int blocks, int waves_per_block, int rows_per_lane);
 
 /* Device bytes held by the handle. */

@@ -24,7 +24,7 @@ def case(xdv=8, ydv=8, zdv=16, precision=dnsgpu.FP64):
 
 @pytest.mark.parametrize("grid,why", [
     ((8, 8, 3), "4<=zdv"),            # fewer fluid rows than the wall rule needs
-    ((8, 8, 513), "4<=zdv"),          # more rows than one block's waves cover
+    ((8, 8, 1025), "4<=zdv"),         # more planes than the finisher's block covers
     ((1, 8, 16), "xdv,ydv>=2"),
     ((8, 1, 16), "xdv,ydv>=2"),
     ((0, 0, 0), "xdv,ydv>=2"),

@@ -155,7 +155,7 @@ def test_tall_member_matches_single_case_path():
     """A member taller than 292 rows makes the step kernel read the plane
     shifts from global memory instead of LDS: the same bitwise results
     against the single-case path (constant F).  A case above 512 rows has
-    no footprint (neither path takes it)."""
+    no member footprint (the ensemble does not take it)."""
     print("case: tall member", flush=True)
     n, F = 12, 2e-5
     ens = dnsgpu.Ensemble(precision=dnsgpu.FP64)

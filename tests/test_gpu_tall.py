@@ -1,0 +1,205 @@
+"""Columns taller than 512 rows (up to zdv = 1024): the step kernel splits
+each column into z-pieces of at most 512 rows, one group of blocks per
+piece, and the moments and statistics kernels take the piece from their
+grid's y index.  The same parity bars as the shorter columns, against the
+oracle (pinned bitwise to the reference):
+  * constant F, FP64: populations bitwise equal after N steps, for two
+    launch shapes (the state does not depend on the grid); log rows within
+    1e-10 relative;
+  * constant Q, FP64: populations within 1e-12 relative, log rows within
+    1e-10; the first step's force comes from the moments kernel; two-pass
+    statistics within 1e-10 and the fields bitwise;
+  * riblets (solid bands, read-side bounce-back), constant F: bitwise;
+  * FP32-shifted storage with riblets, constant Q, against FP64: the bars
+    of the shorter columns (1e-5 in the populations, 1e-3 of the force).
+The step grid is a multiple of the piece count; two rows per lane (FP16)
+takes one piece only and falls back to one row; the ensemble does not take
+members above 512 rows.
+"""
+import numpy as np
+import pytest
+
+import dnsgpu
+import fields
+import pyoracle
+
+pytestmark = pytest.mark.gpu
+
+
+def rel(a, b):
+    return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def seeded(ch, uc=0.06, amp=0.05, seed=7):
+    ch.init_field(dnsgpu.INIT_POISEUILLE, uc=uc, amp=amp, seed=seed)
+    return fields.device_poiseuille(ch.X, ch.Y, ch.Z, uc, amp, seed)
+
+
+def walls_of(X, Y, walls):
+    if walls == "posts":
+        m = fields.posts_mask(X, Y)
+        return m, m
+    if walls == "stripes":
+        return fields.stripes_mask(X, Y, 6, 3), fields.stripes_mask(X, Y, 4, 1, offset=2)
+    return None, None
+
+
+def make(X, Y, Z, forcing, F=0.0, walls="smooth", precision=dnsgpu.FP64, Rebs=60.0, ub=0.05, CFL=0.25,
+         TSTR=100):
+    ch = dnsgpu.Channel(X, Y, Z, Rebs=Rebs, ub=ub, CFL=CFL, TSTR=TSTR, forcing=forcing, force=F,
+                        precision=precision)
+    bot, top = walls_of(X, Y, walls)
+    if bot is not None:
+        ch.set_walls(bot, top)
+    o = pyoracle.Oracle(X, Y, Z, pyoracle.omega(Rebs, ub, Z), ub, CFL, TM=TSTR,
+                        forcing=1 if forcing == dnsgpu.CONST_F else 0, F=F,
+                        mask=pyoracle.wall_mask(X, Y, Z, bot, top))
+    return ch, o
+
+
+LOG_KEYS = ("K", "flowrate", "massrate", "stup", "stdown", "stress", "pgrad", "rho")
+
+
+@pytest.mark.parametrize("shape,walls", [
+    ((4, 4, 640), "posts"),      # 10 chunks: two pieces of 5
+    ((3, 4, 1024), "stripes"),   # the tallest column: two pieces of 8
+    ((2, 3, 577), "smooth"),     # a one-row last chunk in the second piece
+])
+def test_tall_const_force_fp64_bitwise(shape, walls):
+    X, Y, Z = shape
+    n = 60
+    ostate = None
+    for blocks in (0, 6):  # the library's default grid, then 6 blocks of one wave
+        print(f"case: tall const-F FP64 {shape} {walls}, grid {blocks or 'default'}", flush=True)
+        ch, o = make(X, Y, Z, dnsgpu.CONST_F, F=2e-5, walls=walls)
+        f0 = seeded(ch)
+        if blocks:
+            ch.set_step_shape(blocks, 1)
+        b, w = ch.step_shape()
+        assert b % 2 == 0, (b, w)  # a multiple of the two pieces
+        logs = ch.step(n, cntr0=0)
+        ft, f = ch.save_aos()
+        ch.close()
+        if ostate is None:
+            oft, of = f0.copy(), f0.copy()
+            ologs = o.run(oft, of, n)
+            ostate = (oft, of, ologs)
+        oft, of, ologs = ostate
+        assert np.array_equal(ft, oft), rel(ft, oft)
+        assert np.array_equal(f[:, :, 1:Z + 1], of[:, :, 1:Z + 1])
+        for k in LOG_KEYS:
+            assert np.allclose(logs[k], ologs[k], rtol=1e-10, atol=1e-300), (k, logs[k][:3], ologs[k][:3])
+        assert np.array_equal(logs["t"], ologs["t"])
+
+
+def test_tall_const_q_fp64_statistics_and_fields():
+    X, Y, Z = 4, 4, 1024
+    print(f"case: tall const-Q FP64 {X}x{Y}x{Z} posts", flush=True)
+    ch, o = make(X, Y, Z, dnsgpu.CONST_Q, walls="posts")
+    f0 = seeded(ch)
+    n = 100
+    logs = ch.step(n)
+    ft, _ = ch.save_aos()
+    oft, of = f0.copy(), f0.copy()
+    ologs = o.run(oft, of, n)
+    assert rel(ft, oft) <= 1e-12
+    for k in LOG_KEYS + ("force",):
+        scale = np.abs(ologs[k]).max()
+        assert np.abs(logs[k] - ologs[k]).max() <= 1e-10 * scale, k
+    st = ch.statistics()
+    assert np.allclose(st, o.statistics(ft), rtol=1e-10, atol=1e-15)
+    vel, den = ch.fields()
+    ovel, oden = o.moments(ft)
+    assert np.array_equal(vel, ovel) and np.array_equal(den, oden)
+    ch.close()
+
+
+def make_riblets(X, Y, Z, forcing, F=0.0, precision=dnsgpu.FP64):
+    ch = dnsgpu.Channel(X, Y, Z, Rebs=60.0, ub=0.05, CFL=0.25, TSTR=5, forcing=forcing, force=F,
+                        precision=precision)
+    hb = fields.riblet_bands(X, Y, 8, 5)
+    ch.set_solid_bands(hb, hb)
+    s = fields.band_mask(X, Y, Z, hb, hb)
+    o = pyoracle.Oracle(X, Y, Z, pyoracle.omega(60.0, 0.05, Z), 0.05, 0.25, TM=5,
+                        forcing=1 if forcing == dnsgpu.CONST_F else 0, F=F, mask=s)
+    return ch, o, s
+
+
+def test_tall_riblets_const_force_fp64_bitwise():
+    X, Y, Z = 4, 16, 1024
+    print(f"case: tall riblets const-F FP64 {X}x{Y}x{Z} s=8 h=5", flush=True)
+    ch, o, s = make_riblets(X, Y, Z, dnsgpu.CONST_F, F=2e-5)
+    f0 = seeded(ch)
+    f0[s != 0] = 0.0
+    n = 40
+    logs = ch.step(n)
+    ft, _ = ch.save_aos()
+    oft, of = f0.copy(), f0.copy()
+    ologs = o.run(oft, of, n)
+    assert np.array_equal(ft, oft), rel(ft, oft)
+    for k in ("K", "flowrate", "massrate", "stup", "stdown", "stress", "rho"):
+        assert np.allclose(logs[k], ologs[k], rtol=1e-10, atol=1e-300), k
+    ch.close()
+
+
+def test_tall_riblets_fp32_against_fp64():
+    X, Y, Z = 4, 16, 1024
+    print(f"case: tall riblets const-Q FP32 vs FP64 {X}x{Y}x{Z}", flush=True)
+    a, _, s = make_riblets(X, Y, Z, dnsgpu.CONST_Q)
+    b, _, _ = make_riblets(X, Y, Z, dnsgpu.CONST_Q, precision=dnsgpu.FP32)
+    seeded(a)
+    seeded(b)
+    la = a.step(300)
+    lb = b.step(300)
+    fa = a.save_aos()[0]
+    fb = b.save_aos()[0]
+    assert np.abs(fa - fb).max() <= 1e-5
+    assert np.abs(la["force"] - lb["force"]).max() <= 1e-3 * np.abs(la["force"]).max()
+    assert np.all(fb[s != 0] == 0.0)
+    a.close()
+    b.close()
+
+
+def test_tall_launch_shapes():
+    """The grid stays a multiple of the piece count for any request and the
+    autotuner keeps it so; two rows per lane (FP16) and more waves than the
+    LDS of a piece allows are refused; every shape gives the same state
+    (constant F)."""
+    X, Y, Z = 6, 4, 1000
+    print(f"case: tall launch shapes FP16 {X}x{Y}x{Z}", flush=True)
+    ref = None
+    for shape in ((7, 3), (1, 1), (5, 2), None):
+        ch = dnsgpu.Channel(X, Y, Z, Rebs=60.0, ub=0.05, CFL=0.25, forcing=dnsgpu.CONST_F, force=2e-5,
+                            precision=dnsgpu.FP16)
+        seeded(ch)
+        if shape is None:
+            ch.autotune(trial_steps=2)
+            with pytest.raises(dnsgpu.DnsGpuError, match="rows per lane"):
+                ch.set_step_shape(8, 2, 2)
+            with pytest.raises(dnsgpu.DnsGpuError, match="waves"):
+                ch.set_step_shape(8, 4, 1)  # 4 x 16 KiB of plane sums + the block's globals > 64 KiB
+        else:
+            ch.set_step_shape(*shape)
+        b, w, r = ch.step_shape(rows=True)
+        print(f"requested {shape}: {b} blocks x {w} waves, {r} rows per lane", flush=True)
+        assert b >= 2 and b % 2 == 0 and r == 1
+        if shape is not None:
+            assert b == max(2, shape[0] // 2 * 2) and w == shape[1]
+        ch.step(20, logs=False)
+        ft = ch.save_aos()[0]
+        ch.close()
+        if ref is None:
+            ref = ft
+        assert np.array_equal(ft, ref)
+
+
+def test_ensemble_refuses_tall_member():
+    print("case: ensemble with a 640-row member", flush=True)
+    assert dnsgpu.member_footprint(4, 4, 640, dnsgpu.FP32) == 0
+    ens = dnsgpu.Ensemble(precision=dnsgpu.FP32)
+    ch = dnsgpu.Channel(4, 4, 640, Rebs=60.0, ub=0.05, CFL=0.25, precision=dnsgpu.FP32)
+    ch.init_rest()
+    with pytest.raises(dnsgpu.DnsGpuError, match="above the ensemble"):
+        ens.add(ch)
+    ens.close()
+    ch.close()
