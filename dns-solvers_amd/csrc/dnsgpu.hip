@@ -656,7 +656,8 @@ static int place_pair(dnsgpu* h, int trial)
 {
     const size_t bytes = (size_t)h->g.total * h->esize;
     const char* env = getenv("DNSGPU_PAIR_PLACEMENT");
-    if ((env && !strcmp(env, "0")) || bytes < kPairMinBytes || h->pair_candidates) return 0;
+    // (pairs below kPairMinBytes may be arena ranges, which are never freed here)
+    if ((env && !strcmp(env, "0")) || bytes < kPairMinBytes || h->pair_arena || h->pair_candidates) return 0;
     auto time_pair = [&](double* t) {
         double a = 0.0, b = 0.0;
         int rc = time_shape(h, trial, &a);

@@ -9,8 +9,8 @@
 // their buffers, allocated one after the other, share a half.  A caller that
 // is about to create many small cases (the request loop) reserves pair
 // arenas on the device (dnsgpu_reserve_pair_arenas): 8 GiB blocks, each
-// classed into one half when it is allocated, until both halves hold the
-// bytes asked for; while they exist, dnsgpu_create takes a small case's
+// classed into one half when it is allocated, until each half has the
+// bytes asked for free; while they exist, dnsgpu_create takes a small case's
 // buf[0] from a block of one half and buf[1] from a block of the other
 // (first fit, 2 MiB granules), and falls back to plain allocations when
 // they are full.  Measured on the 64-request sweep pool with all members
@@ -138,11 +138,12 @@ int grow(DeviceArenas& d, hipStream_t s)
     return 1;
 }
 
-size_t capacity(const DeviceArenas& d, int half)
+// Bytes of the half's blocks not held by live ranges.
+size_t free_bytes(const DeviceArenas& d, int half)
 {
     size_t n = 0;
     for (const Block* b : d.blocks)
-        if (b->half == half) n += b->ra.capacity();
+        if (b->half == half) n += b->ra.capacity() - b->ra.used();
     return n;
 }
 
@@ -226,16 +227,16 @@ extern "C" int dnsgpu_reserve_pair_arenas(size_t bytes_per_half, size_t* reserve
     size_t free_b = 0, total_b = 0;
     if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) total_b = 0;
     const int max_tries = (int)(total_b / kBlock) + 4;
-    for (int tries = 0; tries < max_tries && (capacity(d, 0) < want || capacity(d, 1) < want); tries++)
+    for (int tries = 0; tries < max_tries && (free_bytes(d, 0) < want || free_bytes(d, 1) < want); tries++)
         if (grow(d, s) == 0) break;
     (void)hipStreamDestroy(s);
     // the half reached first holds more than asked: its empty surplus goes
     for (int half = 0; half < 2; half++)
         for (size_t i = d.blocks.size(); i-- > 0;) {
             Block* b = d.blocks[i];
-            if (b->half == half && b->ra.empty() && capacity(d, half) >= want + kBlock) free_block(d, i);
+            if (b->half == half && b->ra.empty() && free_bytes(d, half) >= want + kBlock) free_block(d, i);
         }
-    if (reserved_per_half) *reserved_per_half = std::min(capacity(d, 0), capacity(d, 1));
+    if (reserved_per_half) *reserved_per_half = std::min(free_bytes(d, 0), free_bytes(d, 1));
     return 0;
 }
 

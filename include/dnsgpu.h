@@ -333,9 +333,10 @@ int dnsgpu_pair_placement(const dnsgpu *h, int *candidates, double *step_gain);
 
 /* Pair arenas for many small cases (ensemble members).  Reserves on the
  * current device 8 GiB blocks, each classed by a timed copy probe into one
- * of the two halves of device memory, until both halves hold
- * bytes_per_half (as device memory allows; *reserved_per_half receives what
- * each half holds, may be NULL).  While they exist, dnsgpu_create takes the
+ * of the two halves of device memory, until each half has bytes_per_half
+ * not held by live cases (as device memory allows; *reserved_per_half
+ * receives the smaller of the two halves' free bytes, may be NULL).  While
+ * they exist, dnsgpu_create takes the
  * two population buffers of a case below 2 GiB per buffer as ranges of a
  * block of each half (first fit, 2 MiB granules) and falls back to plain
  * allocations when they are full; a destroyed handle returns its ranges.
