@@ -83,6 +83,7 @@ def run_precision(wname, prec, steps, warmup):
             ch.init_field(dnsgpu.INIT_TURBULENT, uc=w["utau"], amp=0.1, seed=2, nmodes=8)
         blocks, _ = ch.autotune(trial_steps=5)
         _, waves, rows = ch.step_shape(rows=True)
+        pieces = ch.step_pieces()
         placement = ch.pair_placement()
         ch.step(warmup, cntr0=0, logs=True)
         t0 = time.perf_counter()
@@ -113,6 +114,7 @@ def run_precision(wname, prec, steps, warmup):
         "step_blocks": blocks,
         "step_waves_per_block": waves,
         "step_rows_per_lane": rows,
+        "step_z_pieces": pieces,
         "pair_placement": {"destinations_tried": placement[0], "step_gain": round(placement[1], 4)},
         "stress_last": float(logs["stress"][-1]),
     }
@@ -727,19 +729,22 @@ def run_placement_sweep(args):
 
 
 def _step_variant(name):
-    """(precision, FIRST, STATS, rows per lane) of a step_kernel dispatch name
-    as rocprofv3 writes it (demangled, or mangled for the FP16 instances),
-    or None for any other kernel."""
+    """(precision, FIRST, STATS, rows per lane, z-pieces) of a step_kernel
+    dispatch name as rocprofv3 writes it (demangled, or mangled for the FP16
+    instances), or None for any other kernel.  z-pieces is True for the
+    instance that splits columns into z-pieces (its launch's grid then holds
+    one group of blocks per piece)."""
     import re
-    m = re.match(r"void lbm::step_kernel<(\w+), (\w+), (\w+), (\w+), (\d)>", name)
+    m = re.match(r"void lbm::step_kernel<(\w+), (\w+), (\w+), (\w+), (\d)(?:, (\w+))?>", name)
     if m:
-        t, first, stats, _, rows = m.groups()
+        t, first, stats, _, rows, pieces = m.groups()
         return ({"double": "fp64", "float": "fp32", "_Float16": "fp16"}[t], first == "true", stats == "true",
-                int(rows))
-    m = re.match(r"_ZN3lbmL11step_kernelI(d|f|DF16_)Lb(\d)ELb(\d)ELb(\d)ELi(\d)E", name)
+                int(rows), pieces == "true")
+    m = re.match(r"_ZN3lbmL11step_kernelI(d|f|DF16_)Lb(\d)ELb(\d)ELb(\d)ELi(\d)E(?:Lb(\d)E)?", name)
     if m:
-        t, first, stats, _, rows = m.groups()
-        return {"d": "fp64", "f": "fp32", "DF16_": "fp16"}[t], first == "1", stats == "1", int(rows)
+        t, first, stats, _, rows, pieces = m.groups()
+        return ({"d": "fp64", "f": "fp32", "DF16_": "fp16"}[t], first == "1", stats == "1", int(rows),
+                pieces == "1")
     return None
 
 
@@ -770,10 +775,11 @@ def run_traffic_summary(args):
             v = _step_variant(r["Kernel_Name"])
             if v is None:
                 continue
-            pname, first, stats, rows = v
+            pname, first, stats, rows, pieces = v
             key = f"{pname}_{'first_' if first else ''}{'stats' if stats else 'nostats'}"
             wg = int(r["Workgroup_Size"])
-            shape = f"{int(r['Grid_Size']) // wg}x{wg}" + ("x2rows" if rows == 2 else "")
+            shape = f"{int(r['Grid_Size']) // wg}x{wg}" + ("x2rows" if rows == 2 else "") + \
+                ("xpieces" if pieces else "")
             ent = sums.setdefault(key, {}).setdefault(shape, {"fetch": [], "write": []})
             ent[kind].append(float(r["Counter_Value"]))
             last[(key, kind)] = shape  # the logged steps come last
@@ -907,6 +913,8 @@ def main():
         shape = f"{runs[p]['step_blocks']}x{64 * runs[p]['step_waves_per_block']}"
         if runs[p]["step_rows_per_lane"] == 2:
             shape += "x2rows"
+        if runs[p].get("step_z_pieces", 1) > 1:
+            shape += "xpieces"
         by = (ent.get("by_shape") or {}).get(shape)
         return (by or ent).get("traffic_bytes_per_launch"), shape if by else ent.get("chosen_shape")
 
@@ -914,7 +922,9 @@ def main():
     traffic = {p: t for p, (t, _) in traffic_shape.items()}
     tname = {dnsgpu.FP64: "double", dnsgpu.FP32: "float", dnsgpu.FP16: "_Float16"}
     bands = "true" if w.get("riblets") else "false"
-    kname = {p: f"step_kernel<{tname[p]},false,true,{bands}" + (",2>" if runs[p]["step_rows_per_lane"] == 2 else ">")
+    kname = {p: f"step_kernel<{tname[p]},false,true,{bands}" +
+             (",2>" if runs[p]["step_rows_per_lane"] == 2 else ",1,true>" if runs[p].get("step_z_pieces", 1) > 1
+              else ">")
              for p in runs}
     r32 = runs[dnsgpu.FP32]
     r64 = runs.get(dnsgpu.FP64)

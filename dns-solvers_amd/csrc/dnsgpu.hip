@@ -75,15 +75,16 @@ static bool step_rows_ok(int precision, int Z)
     return precision == DNSGPU_FP16 && Z >= 2 && Z % 2 == 0 && Z <= kWave * kMaxWaves;
 }
 
-// Blocks of the step kernel per CU at W waves per block and `rows` rows per
-// lane (the statistics variant with bands: the largest register set).
-static hipError_t step_occupancy(int precision, int Z, int W, int rows, int* per)
+// Blocks of the step kernel per CU at W waves per block, `rows` rows per
+// lane and nzp z-pieces (the statistics variant with bands: the largest
+// register set).
+static hipError_t step_occupancy(int precision, int Z, int W, int rows, int nzp, int* per)
 {
-    const size_t lds = step_lds_bytes(Z, W, true);
+    const size_t lds = step_lds_bytes(Z, nzp, W, true);
     if (rows == 2)
         return hipOccupancyMaxActiveBlocksPerMultiprocessor(per, step_kernel<_Float16, false, true, true, 2>,
                                                             kWave * W, lds);
-    const bool pieces = z_pieces((Z + kWave - 1) / kWave) > 1;
+    const bool pieces = nzp > 1;
     return with_storage(precision, [&](auto t) {
         if (pieces)
             return hipOccupancyMaxActiveBlocksPerMultiprocessor(
@@ -206,8 +207,11 @@ static void enqueue_step_kernel(dnsgpu* h, bool stats)
     const void* src = h->buf[h->cur];
     void* dst = h->buf[1 - h->cur];
     StepArgs A = step_args(h, src, dst, h->step_nblk);
+    // the step's own z-pieces (a launch-shape choice)
+    A.g.nzp = h->step_nzp;
+    A.g.cpp = (h->g.nzc + h->step_nzp - 1) / h->step_nzp;
     dim3 grid(h->step_nblk), block(kWave * h->step_w);
-    const size_t lds = step_lds_bytes(h->g.Z, h->step_w, stats);
+    const size_t lds = step_lds_bytes(h->g.Z, h->step_nzp, h->step_w, stats);
     if constexpr (std::is_same<T, _Float16>::value) {
         if (h->step_rows == 2 && !h->poststream) {
 #define STEP_R2(S, B) hipLaunchKernelGGL((step_kernel<T, false, S, B, 2>), grid, block, lds, h->stream, A)
@@ -224,7 +228,7 @@ static void enqueue_step_kernel(dnsgpu* h, bool stats)
     }
 #define STEP_VARIANT(F, S, B)                                                                      \
     do {                                                                                           \
-        if (h->g.nzp > 1)                                                                          \
+        if (h->step_nzp > 1)                                                                       \
             hipLaunchKernelGGL((step_kernel<T, F, S, B, 1, true>), grid, block, lds, h->stream, A); \
         else                                                                                       \
             hipLaunchKernelGGL((step_kernel<T, F, S, B>), grid, block, lds, h->stream, A);         \
@@ -256,7 +260,7 @@ static int launch_step(dnsgpu* h, bool stats, LogRow* row, long long cntr)
     const bool rs = std::is_same<T, _Float16>::value && h->step_rows == 2 && !h->poststream && h->has_bands;
     enqueue_step_kernel<T>(h, stats);
     HIPCHK(hipGetLastError());
-    int rc = reduce_partials(h, stats, h->step_nblk / h->g.nzp, h->step_nblk);
+    int rc = reduce_partials(h, stats, h->step_nblk / h->step_nzp, h->step_nblk);
     if (rc) return rc;
     FinalizeArgs F = fin_args(h, stats, row, cntr, 0);
     hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1024), 0, h->stream, F);
@@ -308,13 +312,42 @@ static int update_wall_words(dnsgpu* h)
     return 0;
 }
 
-// A step grid of about nblk blocks: a positive multiple of the z-piece
-// count (each group of nzp consecutive blocks updates all pieces of its
-// columns).
+// A step grid of about nblk blocks: a positive multiple of the step's
+// z-piece count (each group of nzp consecutive blocks updates all pieces of
+// its columns).
 static int step_grid(const dnsgpu* h, int nblk)
 {
-    const int P = h->g.nzp;
+    const int P = h->step_nzp;
     return std::max(P, nblk / P * P);
+}
+
+// The step kernel's z-piece counts for this case: the column's own pieces
+// (z_pieces), and two where that is one and the column has 6 or 8 chunks
+// (321-384 or 449-512 rows: two equal pieces).  Two pieces halve each wave's
+// plane-sum slice, so more waves fit in the LDS; each piece's blocks also
+// read the column's records and the pulled lines at the piece boundary.
+// Same-handle runs at 1536x768x384 with riblets: FP64 27.4-27.5 -> 26.2 ms,
+// FP16 8.07-8.11 -> 7.72 ms, FP32 13.13 -> 13.35-13.41 ms; three or six
+// pieces slower than two in every storage.  The default is two for FP64 and
+// FP16 storage, one for FP32; the autotuner times both.
+static int step_piece_options(const dnsgpu* h, int* opts)
+{
+    const int base = h->g.nzp;
+    opts[0] = base;
+    if (base == 1 && (h->g.nzc == 6 || h->g.nzc == 8)) {
+        opts[1] = 2;
+        return 2;
+    }
+    return 1;
+}
+
+static bool step_piece_ok(const dnsgpu* h, int nzp)
+{
+    int opts[2];
+    const int n = step_piece_options(h, opts);
+    for (int i = 0; i < n; i++)
+        if (opts[i] == nzp) return true;
+    return false;
 }
 
 // Geometry, constants and launch grids of a case (no allocation).
@@ -340,12 +373,23 @@ static int plan_handle(const dns_case* c, dnsgpu* h)
     g.ngroups = (g.ncols + g.cpb - 1) / g.cpb;
     g.total = (long long)g.ncols * Q * g.Z;
     h->block = kWave * g.cpp * g.cpb;
-    h->step_w = step_waves(g.Z);
+    {
+        int opts[2];
+        const int n = step_piece_options(h, opts);
+        h->step_nzp = n > 1 && c->precision != DNSGPU_FP32 ? opts[1] : opts[0];
+        const char* spe = getenv("DNSGPU_STEP_PIECES");
+        if (spe && step_piece_ok(h, atoi(spe))) h->step_nzp = atoi(spe);
+    }
+    h->step_w = step_waves(g.Z, h->step_nzp);
     const char* swe = getenv("DNSGPU_STEP_WAVES");
     if (swe) h->step_w = std::max(1, std::min(atoi(swe), h->step_w));
     h->step_rows = 1;
     const char* sre = getenv("DNSGPU_STEP_ROWS");
-    if (sre && atoi(sre) == 2 && step_rows_ok(c->precision, g.Z)) h->step_rows = 2;
+    if (sre && atoi(sre) == 2 && step_rows_ok(c->precision, g.Z)) {
+        h->step_rows = 2;
+        h->step_nzp = 1;  // two rows per lane: one piece
+        h->step_w = step_waves(g.Z, 1);
+    }
     h->esize = storage_bytes(c->precision);
     // omega and the constants exactly as dns-const-q-xycut.c:211-217,
     // wshearstr.c:24, encalc.c:20 evaluate them.
@@ -388,15 +432,16 @@ static int plan_handle(const dns_case* c, dnsgpu* h)
     int sb = (ncu * waves_per_cu + h->step_w - 1) / h->step_w;
     // never more blocks than can be resident at once (the grid loops)
     int per = 0;
-    HIPCHK(step_occupancy(c->precision, g.Z, h->step_w, h->step_rows, &per));
+    HIPCHK(step_occupancy(c->precision, g.Z, h->step_w, h->step_rows, h->step_nzp, &per));
     h->step_resident = std::max(1, per) * ncu;
     sb = std::min(sb, h->step_resident);
     const char* sbe = getenv("DNSGPU_STEP_BLOCKS");
     if (sbe) sb = atoi(sbe);
     // the partial-sum buffers hold one row per block for any block shape
     // the autotuner may choose (down to one wave per block); the step grid
-    // is a multiple of the z-piece count
-    h->step_nblk_max = std::max(g.nzp, std::min(g.ncols * g.nzp, 64 * ncu) / g.nzp * g.nzp);
+    // is a multiple of its z-piece count (at most 2 * g.nzp)
+    const int pmax = 2 * g.nzp;
+    h->step_nblk_max = std::max(pmax, std::min(g.ncols * pmax, 64 * ncu) / pmax * pmax);
     h->step_nblk = step_grid(h, std::min(sb, h->step_nblk_max));
     return 0;
 }
@@ -548,7 +593,7 @@ struct TuneKey {
 // threads): guarded
 // a tuned launch shape: blocks of the grid and waves per block
 struct TuneShape {
-    int blocks, waves, rows;
+    int blocks, waves, rows, pieces;  // pieces 0: the case's default
 };
 std::vector<std::pair<TuneKey, TuneShape>> g_tune_cache;
 std::mutex g_tune_mutex;
@@ -568,12 +613,16 @@ bool tune_lookup(const TuneKey& k, TuneShape* out)
     TuneKey r;
     TuneShape sh;
     bool found = false;
-    while (fscanf(f, "%d %d %d %d %d %d %d %d", &r.X, &r.Y, &r.Z, &r.prec, &r.bands, &sh.blocks, &sh.waves,
-                  &sh.rows) == 8)
-        if (r == k) {
+    char line[256];
+    while (fgets(line, sizeof line, f)) {
+        sh.pieces = 0;  // lines without the ninth field: the default pieces
+        const int n = sscanf(line, "%d %d %d %d %d %d %d %d %d", &r.X, &r.Y, &r.Z, &r.prec, &r.bands, &sh.blocks,
+                             &sh.waves, &sh.rows, &sh.pieces);
+        if (n >= 8 && r == k) {
             *out = sh;
             found = true;
         }
+    }
     fclose(f);
     if (found) g_tune_cache.push_back({k, *out});
     return found;
@@ -587,24 +636,29 @@ void tune_store(const TuneKey& k, TuneShape sh)
     if (!path) return;
     FILE* f = fopen(path, "a");
     if (!f) return;
-    fprintf(f, "%d %d %d %d %d %d %d %d\n", k.X, k.Y, k.Z, k.prec, k.bands, sh.blocks, sh.waves, sh.rows);
+    fprintf(f, "%d %d %d %d %d %d %d %d %d\n", k.X, k.Y, k.Z, k.prec, k.bands, sh.blocks, sh.waves, sh.rows,
+            sh.pieces);
     fclose(f);
 }
 }  // namespace
 
 // Sets the step kernel's launch shape (W waves per block, nblk blocks,
-// rows per lane) within what the LDS, the device and the partial buffers
-// allow.
-static int set_step_shape(dnsgpu* h, int W, int nblk, int rows)
+// rows per lane, z-pieces; pieces 0 keeps the current count) within what the
+// LDS, the device and the partial buffers allow.  Two rows per lane take
+// one piece.
+static int set_step_shape(dnsgpu* h, int W, int nblk, int rows, int pieces)
 {
-    W = std::max(1, std::min(W, step_waves(h->g.Z)));
     rows = rows == 2 && step_rows_ok(h->c.precision, h->g.Z) ? 2 : 1;
+    int nzp = pieces > 0 && step_piece_ok(h, pieces) ? pieces : h->step_nzp;
+    if (rows == 2) nzp = 1;
+    W = std::max(1, std::min(W, step_waves(h->g.Z, nzp)));
     int per = 0;
-    HIPCHK(step_occupancy(h->c.precision, h->g.Z, W, rows, &per));
+    HIPCHK(step_occupancy(h->c.precision, h->g.Z, W, rows, nzp, &per));
     h->step_w = W;
     h->step_rows = rows;
+    h->step_nzp = nzp;
     h->step_resident = std::max(1, per) * h->ncu;
-    const int cols = std::max(1, (h->g.ncols + W - 1) / W) * h->g.nzp;
+    const int cols = std::max(1, (h->g.ncols + W - 1) / W) * nzp;
     h->step_nblk = step_grid(h, std::min(nblk, std::min(std::min(h->step_nblk_max, h->step_resident), cols)));
     return 0;
 }
@@ -720,33 +774,34 @@ extern "C" int dnsgpu_autotune(dnsgpu* h, int trial_steps, int* best_blocks, dou
     // the handle's state flag and shape are restored on every return
     struct Restore {
         dnsgpu* h;
-        int ps, nblk, w, rows;
+        int ps, nblk, w, rows, pieces;
         ~Restore()
         {
             h->poststream = ps;
-            (void)set_step_shape(h, w, nblk, rows);
+            (void)set_step_shape(h, w, nblk, rows, pieces);
         }
-    } restore{h, h->poststream, h->step_nblk, h->step_w, h->step_rows};
+    } restore{h, h->poststream, h->step_nblk, h->step_w, h->step_rows, h->step_nzp};
     h->poststream = 0;
     const TuneKey key{h->g.X, h->g.Y, h->g.Z, h->c.precision, h->has_bands};
     TuneShape cached;
     const bool have = tune_lookup(key, &cached);
     // the pair first (at the cached or the current shape), so that the
     // shapes are timed on the buffers the steps will use
-    int rc = have ? set_step_shape(h, cached.waves, cached.blocks, cached.rows) : 0;
+    int rc = have ? set_step_shape(h, cached.waves, cached.blocks, cached.rows, cached.pieces) : 0;
     if (!rc) rc = place_pair(h, trial_steps);
     if (rc) return rc;
     if (have) {
         restore.nblk = h->step_nblk;
         restore.w = h->step_w;
         restore.rows = h->step_rows;
+        restore.pieces = h->step_nzp;
         if (best_blocks) *best_blocks = h->step_nblk;
         if (best_ms) *best_ms = -1.0;
         return 0;
     }
-    // time the steady-state kernel over the shapes (waves per block x
-    // waves per CU), then the three fastest again, alternating, to keep a
-    // noisy first round from choosing
+    // time the steady-state kernel over the shapes (rows per lane x
+    // z-pieces x waves per block x waves per CU), then the three fastest
+    // again, alternating, to keep a noisy first round from choosing
     const int wpcs[] = {2, 3, 4, 6, 8, 12, 16, 20, 24, 32};
     struct Cand {
         TuneShape s;
@@ -754,18 +809,23 @@ extern "C" int dnsgpu_autotune(dnsgpu* h, int trial_steps, int* best_blocks, dou
     };
     std::vector<Cand> cands;
     const int max_rows = step_rows_ok(h->c.precision, h->g.Z) ? 2 : 1;
+    int popts[2];
+    const int npopt = step_piece_options(h, popts);
     for (int R = 1; R <= max_rows; R++)
-        for (int W = 2; W <= step_waves(h->g.Z); W++) {
-            int last = -1;
-            for (int wpc : wpcs) {
-                rc = set_step_shape(h, W, (h->ncu * wpc + W - 1) / W, R);
-                if (rc) return rc;
-                if (h->step_nblk == last || h->step_w != W) continue;
-                last = h->step_nblk;
-                double t = 0.0;
-                rc = time_shape(h, trial_steps, &t);
-                if (rc) return rc;
-                cands.push_back({TuneShape{h->step_nblk, W, R}, t});
+        for (int pi = 0; pi < (R == 2 ? 1 : npopt); pi++) {
+            const int P = R == 2 ? 1 : popts[pi];
+            for (int W = 2; W <= step_waves(h->g.Z, P); W++) {
+                int last = -1;
+                for (int wpc : wpcs) {
+                    rc = set_step_shape(h, W, (h->ncu * wpc + W - 1) / W, R, P);
+                    if (rc) return rc;
+                    if (h->step_nblk == last || h->step_w != W || h->step_nzp != P) continue;
+                    last = h->step_nblk;
+                    double t = 0.0;
+                    rc = time_shape(h, trial_steps, &t);
+                    if (rc) return rc;
+                    cands.push_back({TuneShape{h->step_nblk, W, R, P}, t});
+                }
             }
         }
     if (cands.empty()) return set_err(DNSGPU_EHIP, "autotune: no launch shape");
@@ -774,7 +834,7 @@ extern "C" int dnsgpu_autotune(dnsgpu* h, int trial_steps, int* best_blocks, dou
     for (int i = 0; i < nfin; i++) cands[i].t = 0.0;
     for (int round = 0; round < 2; round++)
         for (int i = 0; i < nfin; i++) {
-            int rc = set_step_shape(h, cands[i].s.waves, cands[i].s.blocks, cands[i].s.rows);
+            int rc = set_step_shape(h, cands[i].s.waves, cands[i].s.blocks, cands[i].s.rows, cands[i].s.pieces);
             if (rc) return rc;
             double t = 0.0;
             rc = time_shape(h, trial_steps, &t);
@@ -788,6 +848,7 @@ extern "C" int dnsgpu_autotune(dnsgpu* h, int trial_steps, int* best_blocks, dou
     restore.nblk = best.blocks;
     restore.w = best.waves;
     restore.rows = best.rows;
+    restore.pieces = best.pieces;
     tune_store(key, best);
     if (best_blocks) *best_blocks = best.blocks;
     if (best_ms) *best_ms = cands[bi].t / 2;
@@ -798,13 +859,14 @@ extern "C" int dnsgpu_set_step_shape(dnsgpu* h, int blocks, int waves_per_block,
 {
     if (!h) return set_err(DNSGPU_EARG, "null handle");
     NOT_MEMBER(h);
-    if (blocks < 1 || waves_per_block < 1 || waves_per_block > step_waves(h->g.Z))
-        return set_err(DNSGPU_EARG, "launch shape %d blocks x %d waves (1..%d waves at zdv %d)", blocks,
-                       waves_per_block, step_waves(h->g.Z), h->g.Z);
+    const int nzp = rows_per_lane == 2 ? 1 : h->step_nzp;  // two rows per lane: one piece
+    if (blocks < 1 || waves_per_block < 1 || waves_per_block > step_waves(h->g.Z, nzp))
+        return set_err(DNSGPU_EARG, "launch shape %d blocks x %d waves (1..%d waves at zdv %d, %d z-pieces)", blocks,
+                       waves_per_block, step_waves(h->g.Z, nzp), h->g.Z, nzp);
     if (rows_per_lane != 1 && !(rows_per_lane == 2 && step_rows_ok(h->c.precision, h->g.Z)))
         return set_err(DNSGPU_EARG, "%d rows per lane (2 needs FP16 storage and an even zdv <= %d)", rows_per_lane,
                        kWave * kMaxWaves);
-    return set_step_shape(h, waves_per_block, blocks, rows_per_lane);
+    return set_step_shape(h, waves_per_block, blocks, rows_per_lane, nzp);
 }
 
 extern "C" int dnsgpu_step_shape(const dnsgpu* h, int* blocks, int* waves_per_block, int* rows_per_lane)
@@ -813,6 +875,26 @@ extern "C" int dnsgpu_step_shape(const dnsgpu* h, int* blocks, int* waves_per_bl
     if (blocks) *blocks = h->step_nblk;
     if (waves_per_block) *waves_per_block = h->step_w;
     if (rows_per_lane) *rows_per_lane = h->step_rows;
+    return 0;
+}
+
+extern "C" int dnsgpu_set_step_pieces(dnsgpu* h, int pieces)
+{
+    if (!h) return set_err(DNSGPU_EARG, "null handle");
+    NOT_MEMBER(h);
+    int opts[2];
+    const int n = step_piece_options(h, opts);
+    if (!step_piece_ok(h, pieces))
+        return set_err(DNSGPU_EARG, "%d z-pieces at zdv %d (%d%s)", pieces, h->g.Z, opts[0],
+                       n > 1 ? " or 2" : " only");
+    if (h->step_rows == 2 && pieces != 1) return set_err(DNSGPU_EARG, "two rows per lane take one z-piece");
+    return set_step_shape(h, h->step_w, h->step_nblk, h->step_rows, pieces);
+}
+
+extern "C" int dnsgpu_step_pieces(const dnsgpu* h, int* pieces)
+{
+    if (!h) return set_err(DNSGPU_EARG, "null handle");
+    if (pieces) *pieces = h->step_nzp;
     return 0;
 }
 
@@ -1514,7 +1596,7 @@ extern "C" int dnsgpu_time_steps(dnsgpu* h, int nsteps, int with_stats, double* 
         with_storage(h->c.precision, [&](auto t) { enqueue_step_kernel<decltype(t)>(h, st); });
         HIPCHK(hipGetLastError());
         HIPCHK(hipEventRecord(ev[2 * s + 1], h->stream));
-        int rc = reduce_partials(h, st, h->step_nblk / h->g.nzp, h->step_nblk);
+        int rc = reduce_partials(h, st, h->step_nblk / h->step_nzp, h->step_nblk);
         if (rc) return rc;
         FinalizeArgs F = fin_args(h, st, nullptr, s, 0);
         hipLaunchKernelGGL(finalize_kernel, dim3(1), dim3(1024), 0, h->stream, F);

@@ -58,6 +58,7 @@ struct dnsgpu {
     int nblk;       // blocks of the chunk-mapped kernels (moments, statistics)
     int step_w;     // waves per block of the step kernel
     int step_rows;  // rows per lane of the step kernel (2: FP16 storage, even Z)
+    int step_nzp;   // z-pieces of the step kernel (g.nzp, or 2 at 6 or 8 chunks)
     int step_nblk;  // blocks of the step kernel
     int ncu;
     int step_nblk_max;  // capacity of the partial buffers for the step grid

@@ -49,13 +49,16 @@ constexpr int kMaxZ = 1024;
 // A column taller than kMaxWaves chunks (512 rows) is split into z-pieces
 // of at most kMaxWaves chunks: the step kernel gives each piece to its own
 // group of blocks (each wave's plane sums then cover one piece), and the
-// chunk-mapped kernels to the y index of their grid.
+// chunk-mapped kernels to the y index of their grid.  The step kernel may
+// split a shorter column too (a launch-shape choice: smaller plane-sum
+// slices, more waves resident); the Geom of the StepArgs it is launched with
+// carries its piece count.
 __host__ __device__ inline int z_pieces(int nzc) { return (nzc + kMaxWaves - 1) / kMaxWaves; }
 
 struct Geom {
     int X, Y, Z;
     int nzc;          // 64-row chunks per column
-    int nzp;          // z-pieces per column (1 up to 512 rows)
+    int nzp;          // z-pieces per column (z_pieces; the step kernel's launch: its piece count)
     int cpp;          // chunks per z-piece: ceil(nzc / nzp)
     int cpb;          // columns per block iteration (chunk-mapped kernels)
     int ncols;        // X*Y
@@ -406,30 +409,30 @@ __device__ __forceinline__ void gather_wave_read_side(const gbyte* const (&sp)[3
 // copy is 3-5% faster).
 __host__ __device__ inline bool step_shifts_in_lds(int Z) { return Z <= 256; }
 
-// Planes of one wave's plane sums in the step kernel: the column, or above
-// 512 rows one z-piece of it (whole chunks).
-__host__ __device__ inline int step_plane_rows(int Z)
+// Planes of one wave's plane sums in the step kernel run with nzp z-pieces:
+// the column, or one piece of it (whole chunks).
+__host__ __device__ inline int step_plane_rows(int Z, int nzp)
 {
-    const int nzc = (Z + kWave - 1) / kWave, P = z_pieces(nzc);
-    return P == 1 ? Z : (nzc + P - 1) / P * kWave;
+    const int nzc = (Z + kWave - 1) / kWave;
+    return nzp == 1 ? Z : (nzc + nzp - 1) / nzp * kWave;
 }
 
 // Dynamic LDS of the step kernel: [W][kNG] per-wave global sums, then with
 // STATS [W][kNQ][PZ] per-wave plane sums (PZ = step_plane_rows) and (see
 // above) the [3][Z] shifts.  At Z = 192, 4 waves: 29 KB, five blocks per CU.
-__host__ __device__ inline size_t step_lds_bytes(int Z, int W, bool stats)
+__host__ __device__ inline size_t step_lds_bytes(int Z, int nzp, int W, bool stats)
 {
-    return sizeof(double) * ((size_t)W * kNG + (stats ? (size_t)W * kNQ * step_plane_rows(Z) +
+    return sizeof(double) * ((size_t)W * kNG + (stats ? (size_t)W * kNQ * step_plane_rows(Z, nzp) +
                                                              (step_shifts_in_lds(Z) ? 3 * (size_t)Z : 0)
                                                        : 0));
 }
 
-// Waves per block of the step kernel for a given Z: at most 4, and the
-// statistics variant's LDS within 64 KiB.
-__host__ __device__ inline int step_waves(int Z)
+// Waves per block of the step kernel for a given Z and z-piece count: at
+// most 4, and the statistics variant's LDS within 64 KiB.
+__host__ __device__ inline int step_waves(int Z, int nzp)
 {
     int w = 4;
-    while (w > 1 && step_lds_bytes(Z, w, true) > 65536) w--;
+    while (w > 1 && step_lds_bytes(Z, nzp, w, true) > 65536) w--;
     return w;
 }
 

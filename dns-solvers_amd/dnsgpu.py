@@ -27,7 +27,8 @@ EXPORTS = (
     "dnsgpu_commit_load", "dnsgpu_save_aos", "dnsgpu_get_force", "dnsgpu_step",
     "dnsgpu_statistics", "dnsgpu_fields", "dnsgpu_time_steps", "dnsgpu_device_bytes",
     "dnsgpu_autotune", "dnsgpu_tavg_reset", "dnsgpu_tavg_sample", "dnsgpu_tavg_get",
-    "dnsgpu_footprint", "dnsgpu_fields_noalloc", "dnsgpu_step_shape", "dnsgpu_set_step_shape", "dnsgpu_set_force",
+    "dnsgpu_footprint", "dnsgpu_fields_noalloc", "dnsgpu_step_shape", "dnsgpu_set_step_shape",
+    "dnsgpu_step_pieces", "dnsgpu_set_step_pieces", "dnsgpu_set_force",
     "dnsgpu_pair_placement", "dnsgpu_reserve_pair_arenas", "dnsgpu_release_pair_arenas", "dnsgpu_pair_arena_stats",
     "dnsens_create", "dnsens_destroy", "dnsens_member_bytes", "dnsens_footprint", "dnsens_add", "dnsens_remove",
     "dnsens_count", "dnsens_bytes_used", "dnsens_budget", "dnsens_bind_thread", "dnsens_step", "dnsens_info", "dnsens_rows",
@@ -125,6 +126,8 @@ def load():
         L.dnsgpu_step_shape.argtypes = [vp, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                         ctypes.POINTER(ctypes.c_int)]
         L.dnsgpu_set_step_shape.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.dnsgpu_step_pieces.argtypes = [vp, ctypes.POINTER(ctypes.c_int)]
+        L.dnsgpu_set_step_pieces.argtypes = [vp, ctypes.c_int]
         L.dnsgpu_set_force.argtypes = [vp, ctypes.c_double]
         L.dnsgpu_time_steps.argtypes = [vp, ctypes.c_int, ctypes.c_int, dp, dp]
         L.dnsgpu_autotune.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int), dp]
@@ -336,6 +339,15 @@ class Channel:
 
     def set_step_shape(self, blocks, waves, rows=1):
         _chk(load().dnsgpu_set_step_shape(self.h, blocks, waves, rows))
+
+    def step_pieces(self):
+        """z-pieces per column of the step kernel's launch."""
+        p = ctypes.c_int()
+        _chk(load().dnsgpu_step_pieces(self.h, ctypes.byref(p)))
+        return p.value
+
+    def set_step_pieces(self, pieces):
+        _chk(load().dnsgpu_set_step_pieces(self.h, pieces))
 
     def tavg_reset(self):
         _chk(load().dnsgpu_tavg_reset(self.h))

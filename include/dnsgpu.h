@@ -308,6 +308,18 @@ int dnsgpu_step_shape(const dnsgpu *h, int *blocks, int *waves_per_block, int *r
  * shape beyond the summation order of the logged sums.  No reference
  * counterpart. */
 int dnsgpu_set_step_shape(dnsgpu *h, int blocks, int waves_per_block, int rows_per_lane);
+/* The step kernel's z-pieces per column: each piece of a column goes to
+ * its own group of blocks, whose waves keep the plane sums of that piece
+ * only.  A column above 512 rows always has pieces of at most 512 rows; a
+ * column of 321-384 or 449-512 rows may run as one piece or two (the
+ * default: two for FP64 and FP16 storage, one for FP32; the autotuner times
+ * both; DNSGPU_STEP_PIECES in the environment sets the default).  Setting
+ * another count is DNSGPU_EARG, and so is more than one piece with two rows
+ * per lane.  The grid is kept a multiple of the count.  The state does not
+ * depend on it; the logged sums only by summation order.  No reference
+ * counterpart. */
+int dnsgpu_step_pieces(const dnsgpu *h, int *pieces);
+int dnsgpu_set_step_pieces(dnsgpu *h, int pieces);
 
 /* Device bytes held by the handle. */
 size_t dnsgpu_device_bytes(const dnsgpu *h);

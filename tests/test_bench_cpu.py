@@ -22,11 +22,20 @@ HDR = ["Correlation_Id", "Dispatch_Id", "Agent_Id", "Queue_Id", "Process_Id", "T
 
 def test_step_variant_names():
     assert bench._step_variant("void lbm::step_kernel<float, false, true, true, 1>(lbm::StepArgs)") == \
-        ("fp32", False, True, 1)
+        ("fp32", False, True, 1, False)
     assert bench._step_variant("void lbm::step_kernel<double, true, true, false, 1>(lbm::StepArgs)") == \
-        ("fp64", True, True, 1)
+        ("fp64", True, True, 1, False)
     assert bench._step_variant("_ZN3lbmL11step_kernelIDF16_Lb0ELb0ELb1ELi2EEEvNS_8StepArgsE") == \
-        ("fp16", False, False, 2)
+        ("fp16", False, False, 2, False)
+    # with the z-piece flag (the kernel's sixth template argument)
+    assert bench._step_variant("void lbm::step_kernel<float, false, true, true, 1, false>(lbm::StepArgs)") == \
+        ("fp32", False, True, 1, False)
+    assert bench._step_variant("void lbm::step_kernel<double, false, true, true, 1, true>(lbm::StepArgs)") == \
+        ("fp64", False, True, 1, True)
+    assert bench._step_variant("_ZN3lbmL11step_kernelIDF16_Lb0ELb1ELb1ELi1ELb1EEEvNS_8StepArgsE") == \
+        ("fp16", False, True, 1, True)
+    assert bench._step_variant("_ZN3lbmL11step_kernelIDF16_Lb0ELb1ELb1ELi2ELb0EEEvNS_8StepArgsE") == \
+        ("fp16", False, True, 2, False)
     assert bench._step_variant("lbm::finalize_kernel(lbm::FinalizeArgs)") is None
 
 

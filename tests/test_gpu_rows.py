@@ -45,6 +45,7 @@ def make(shape, texture, seed=7, forcing=dnsgpu.CONST_F):
 
 
 def two_rows(ch):
+    ch.set_step_pieces(1)  # two rows per lane take one z-piece
     blocks, waves = ch.step_shape()
     ch.set_step_shape(blocks, waves, 2)
     assert ch.step_shape(rows=True)[2] == 2

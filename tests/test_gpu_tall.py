@@ -15,6 +15,10 @@ oracle (pinned bitwise to the reference):
 The step grid is a multiple of the piece count; two rows per lane (FP16)
 takes one piece only and falls back to one row; the ensemble does not take
 members above 512 rows.
+
+Columns of 6 or 8 chunks (321-384, 449-512 rows) may also run as two
+pieces (dnsgpu_set_step_pieces; the default for FP64 and FP16 storage): the
+same state bitwise as one piece and as the oracle, under constant force.
 """
 import numpy as np
 import pytest
@@ -202,4 +206,65 @@ def test_ensemble_refuses_tall_member():
     with pytest.raises(dnsgpu.DnsGpuError, match="above the ensemble"):
         ens.add(ch)
     ens.close()
+    ch.close()
+
+
+@pytest.mark.parametrize("Z,precision,default", [(384, dnsgpu.FP64, 2), (512, dnsgpu.FP32, 1),
+                                                 (384, dnsgpu.FP16, 2)])
+def test_two_pieces_of_a_short_column(Z, precision, default):
+    X, Y = 4, 4
+    n = 40
+    print(f"case: z-pieces {X}x{Y}x{Z} precision {precision}, one and two pieces", flush=True)
+    out = {}
+    for pieces in (1, 2):
+        ch, o = make(X, Y, Z, dnsgpu.CONST_F, F=2e-5, walls="posts", precision=precision)
+        f0 = seeded(ch)
+        assert ch.step_pieces() == default
+        ch.set_step_pieces(pieces)
+        b, w = ch.step_shape()
+        assert ch.step_pieces() == pieces and b % pieces == 0, (b, w)
+        logs = ch.step(n, cntr0=0)
+        out[pieces] = (ch.save_aos()[0], logs)
+        ch.close()
+    assert np.array_equal(out[1][0], out[2][0])
+    for k in LOG_KEYS:
+        assert np.allclose(out[1][1][k], out[2][1][k], rtol=1e-10, atol=1e-300), k
+    if precision == dnsgpu.FP64:
+        oft, of = f0.copy(), f0.copy()
+        o.run(oft, of, n)
+        assert np.array_equal(out[2][0], oft)
+
+
+def test_piece_counts_and_rows_per_lane():
+    print("case: z-piece options", flush=True)
+    for Z, ok in ((320, False), (384, True), (448, False), (512, True), (256, False)):
+        ch = dnsgpu.Channel(4, 4, Z, Rebs=60.0, ub=0.05, CFL=0.25, precision=dnsgpu.FP32)
+        ch.init_rest()
+        if ok:
+            ch.set_step_pieces(2)
+            assert ch.step_pieces() == 2
+        else:
+            with pytest.raises(dnsgpu.DnsGpuError, match="z-pieces"):
+                ch.set_step_pieces(2)
+            assert ch.step_pieces() == 1
+        with pytest.raises(dnsgpu.DnsGpuError, match="z-pieces"):
+            ch.set_step_pieces(3)
+        ch.close()
+    # FP16 two rows per lane take one piece
+    ch = dnsgpu.Channel(8, 8, 384, Rebs=60.0, ub=0.05, CFL=0.25, precision=dnsgpu.FP16)
+    ch.init_rest()
+    assert ch.step_pieces() == 2
+    ch.set_step_shape(8, 2, 2)
+    assert ch.step_shape(rows=True)[2] == 2 and ch.step_pieces() == 1
+    with pytest.raises(dnsgpu.DnsGpuError, match="one z-piece"):
+        ch.set_step_pieces(2)
+    ch.set_step_shape(8, 2, 1)
+    ch.set_step_pieces(2)
+    assert ch.step_pieces() == 2 and ch.step_shape()[0] % 2 == 0
+    # the autotuner keeps a valid combination
+    ch.autotune(trial_steps=2)
+    b, w, r = ch.step_shape(rows=True)
+    p = ch.step_pieces()
+    print(f"autotuned: {b} blocks x {w} waves, {r} rows per lane, {p} pieces", flush=True)
+    assert p in (1, 2) and b % p == 0 and (r == 1 or p == 1)
     ch.close()
