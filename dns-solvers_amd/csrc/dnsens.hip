@@ -178,7 +178,7 @@ static int find_member(const dnsens* e, const dnsgpu* h)
 
 static int tiles_of(const dnsgpu* h)
 {
-    const int cpt = ens_tile_cols(h->g.ncols, h->c.precision == DNSGPU_FP16);
+    const int cpt = ens_tile_cols(h->g.ncols);
     return (h->g.ncols + cpt - 1) / cpt;
 }
 
@@ -194,7 +194,7 @@ extern "C" size_t dnsens_footprint(const dns_case* c)
 {
     const size_t fp = dnsgpu_footprint(c);
     if (!fp || c->zdv > kEnsZmax) return 0;
-    const int ncols = c->xdv * c->ydv, cpt = ens_tile_cols(ncols, c->precision == DNSGPU_FP16);
+    const int ncols = c->xdv * c->ydv, cpt = ens_tile_cols(ncols);
     return fp + sizeof(double) * (size_t)((ncols + cpt - 1) / cpt) * ens_record(c->zdv);
 }
 

@@ -22,17 +22,14 @@ namespace lbm {
 
 constexpr int kEnsWaves = 4;  // waves per block of the ensemble step kernel
 
-// Columns per tile of a member: a function of the member alone (its column
-// count and storage).  Wider tiles write fewer records (kNQ Z doubles each)
-// but lengthen the last round of tiles a launch waits for.  Measured on the
-// 64-solve pool (231.5 M nodes, every member resident), whole step: FP32
-// storage 8/16 columns 6.79-6.85 ms, 16/32 columns 7.06 ms, 32/64 columns
-// 7.46 ms; FP16 storage 8/16 columns 5.01-5.06 ms, 16/32 columns 4.89-5.00 ms.
-__host__ __device__ inline int ens_tile_cols(int ncols, bool half_storage)
-{
-    const int c = ncols >= 32768 ? 16 : 8;
-    return half_storage ? 2 * c : c;
-}
+// Columns per tile of a member: a function of the member alone.  Wider
+// tiles write fewer records (kNQ Z doubles each) but lengthen the last round
+// of tiles a launch waits for and leave a small pool fewer waves.  Measured
+// on the 64-solve pool with every member resident, whole step: FP32 8/16
+// columns 6.79-6.85 ms, 16/32 columns 7.06 ms, 32/64 columns 7.46 ms; FP16
+// 8/16 columns 5.01-5.06 ms, 16/32 columns 4.89-5.00 ms, but the FP16 sweep
+// with Poisson arrivals (a few members resident at a time) busy 62.6 -> 75.9 s.
+__host__ __device__ inline int ens_tile_cols(int ncols) { return ncols >= 32768 ? 16 : 8; }
 
 // Doubles per tile record: kNQ per-plane sums for each of Z planes, then
 // the kNG global sums.

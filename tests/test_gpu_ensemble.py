@@ -250,8 +250,8 @@ def test_admission_by_footprint_and_member_rules():
     assert need == chs[0].device_bytes() + 16 * 44 * 8
     assert dnsgpu.footprint(16, 8, 10, dnsgpu.FP32) == chs[0].device_bytes()
     assert dnsgpu.member_footprint(16, 8, 10, dnsgpu.FP32) == need
-    # FP16 storage: tiles of 16 columns
-    assert dnsgpu.member_footprint(16, 8, 10, dnsgpu.FP16) == dnsgpu.footprint(16, 8, 10, dnsgpu.FP16) + 8 * 44 * 8
+    # FP16 storage: the same tiles of 8 columns
+    assert dnsgpu.member_footprint(16, 8, 10, dnsgpu.FP16) == dnsgpu.footprint(16, 8, 10, dnsgpu.FP16) + 16 * 44 * 8
     ens = dnsgpu.Ensemble(precision=dnsgpu.FP32, budget=2 * need + need // 2)
     ens.add(chs[0])
     ens.add(chs[1])
