@@ -484,33 +484,113 @@ def sweep_measure(prec, burst, sequential=True, kernel=True):
         "total_steps": int(sum(steps)),
     }
     if kernel:
-        # the dominant kernel: the multi-case step with all 64 solves resident,
-        # their population pairs from the pair arenas as in the sweep
-        esize = 4 if prec == dnsgpu.FP32 else 2
-        dnsgpu.reserve_pair_arenas(sum(r.xdv * r.ydv * r.zdv * 19 * esize for r in reqs))
-        hs = [dnsserve.Handle(r, cfg) for r in reqs]
-        ens = dnsgpu.Ensemble(precision=prec)
-        for h in hs:
-            ens.add(h)
-        blocks, _ = ens.autotune(3)
-        ens.step(2)
-        n = 10
-        tk, tw = ens.time_kernels(n)
-        ens.close()
-        for h in hs:
-            h.close()
-        dnsgpu.release_pair_arenas()
-        achieved = nodes * bpn / (tk / n / 1e3) / 1e9
-        tr = measured_traffic(SWEEP)
-        traffic = tr[PNAME[prec]]["traffic_bytes_per_launch"] if tr and PNAME[prec] in tr else None
-        out["ms_per_step_all_resident"] = round(tw / n, 4)
-        out["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                           "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
-                           "traffic_source": f"profiles/traffic_{SWEEP}.json" if traffic else None,
-                           "kernel": "ens_step_kernel<%s>" % ("float" if prec == dnsgpu.FP32 else "_Float16"),
-                           "kernel_ms": round(tk / n, 4), "bytes_per_node": bpn,
-                           "nodes": nodes, "grid_blocks": blocks}
+        pk = pool_kernel_measure(prec)
+        out["ms_per_step_all_resident"] = pk["ms_per_step"]
+        out["roofline"] = pk["roofline"]
     out["per_solve"] = res
+    return out
+
+
+def pool_kernel_measure(prec, n=10):
+    """The serving sweep's dominant kernel: the multi-case step with all 64
+    seeded solves resident, their population pairs from the pair arenas as
+    in the sweep, autotuned, then n steps timed with HIP events on the
+    ensemble's stream (step kernel alone, and the whole step with the record
+    reduction and the finishers)."""
+    import dnsserve
+    reqs = dnsserve.generate(64, seed=5, mean_gap=2.0)
+    cfg = dnsserve.defaults(realtime=0, verbose=0, precision=prec)
+    nodes = sum(r.xdv * r.ydv * r.zdv for r in reqs)
+    bpn = 152 if prec == dnsgpu.FP32 else 76
+    esize = 4 if prec == dnsgpu.FP32 else 2
+    dnsgpu.reserve_pair_arenas(sum(r.xdv * r.ydv * r.zdv * 19 * esize for r in reqs))
+    hs = [dnsserve.Handle(r, cfg) for r in reqs]
+    ens = dnsgpu.Ensemble(precision=prec)
+    for h in hs:
+        ens.add(h)
+    blocks, _ = ens.autotune(3)
+    ens.step(2)
+    tk, tw = ens.time_kernels(n)
+    ens.close()
+    for h in hs:
+        h.close()
+    dnsgpu.release_pair_arenas()
+    achieved = nodes * bpn / (tk / n / 1e3) / 1e9
+    tr = measured_traffic(SWEEP)
+    traffic = tr[PNAME[prec]]["traffic_bytes_per_launch"] if tr and PNAME[prec] in tr else None
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic,
+            "traffic_source": f"profiles/traffic_{SWEEP}.json" if traffic else None,
+            "kernel": "ens_step_kernel<%s>" % ("float" if prec == dnsgpu.FP32 else "_Float16"),
+            "kernel_ms": round(tk / n, 4), "bytes_per_node": bpn, "nodes": nodes, "grid_blocks": blocks,
+            "frac_of_copy_ceiling": round(achieved / CEILING_GBS, 4), "copy_ceiling": CEILING_GBS}
+    if traffic:
+        roof["traffic_gbs"] = round(traffic / (tk / n / 1e3) / 1e9, 1)
+    return {"ms_per_step": round(tw / n, 4), "roofline": roof}
+
+
+def run_pool_only(args):
+    """The multi-case step kernel of the 64-solve sweep with every solve
+    resident (pool_kernel_measure), as its own line."""
+    prec = dnsgpu.FP16 if args.sweep_fp16 else dnsgpu.FP32
+    pk = pool_kernel_measure(prec, n=args.steps)
+    r = pk["roofline"]
+    return {"metric": "multi-case step kernel, 64 solves resident", "value": round(r["nodes"] / r["kernel_ms"] / 1e3, 1),
+            "unit": "MLUPS (step kernel)", "n_gpus": 1, "steps": args.steps, "ms_per_step": pk["ms_per_step"],
+            "higher_is_better": True, "dtype": "f32" if prec == dnsgpu.FP32 else "f32 (f16 storage)",
+            "data": "synthetic", "config": {"workload": SWEEP, "storage": PNAME[prec] + "-shifted"}, "roofline": r}
+
+
+def run_ens_traffic_summary(args):
+    """HBM bytes per dispatch of the multi-case step kernel and the record
+    reduction of the 64-solve pool, from rocprofv3 --pmc runs of bench.py
+    --pool-only (FETCH_SIZE, WRITE_SIZE) for FP32 and FP16 storage, scaled by
+    what the same counters read for the plain copy of exactly 4 GiB (copybw
+    4 2).  Argument: fp32 fetch, fp32 write, copy fetch, copy write, fp16
+    fetch, fp16 write CSVs.  The step kernel's grid is that of its last
+    dispatch (the timed steps, after the autotuner's trials)."""
+    import csv
+    import statistics
+    import dnsserve
+    paths = args.traffic_summary.split(",")
+    f32f, f32w, cf, cw = paths[:4]
+    copy_kib = 4 * 1024 * 1024
+
+    def cal(path):
+        vals = [float(r["Counter_Value"]) for r in csv.DictReader(open(path)) if "copy_f4" in r["Kernel_Name"]]
+        return copy_kib / statistics.median(vals)
+
+    scale = {"fetch": cal(cf), "write": cal(cw)}
+    reqs = dnsserve.generate(64, seed=5, mean_gap=2.0)
+    nodes = sum(r.xdv * r.ydv * r.zdv for r in reqs)
+    out = {"workload": SWEEP, "nodes": nodes, "fetch_scale": round(scale["fetch"], 4),
+           "write_scale": round(scale["write"], 4)}
+    pairs = [("fp32", f32f, f32w)] + ([("fp16", paths[4], paths[5])] if len(paths) >= 6 else [])
+    for pname, fpath, wpath in pairs:
+        per = {}
+        for kind, path in (("fetch", fpath), ("write", wpath)):
+            sums, name, grid = {}, {}, {}
+            for r in csv.DictReader(open(path)):
+                kn = r["Kernel_Name"]
+                if "ens_step_kernel" not in kn and "ens_reduce_kernel" not in kn:
+                    continue
+                d = int(r["Dispatch_Id"])
+                sums[d] = sums.get(d, 0.0) + float(r["Counter_Value"])
+                name[d] = kn
+                grid[d] = int(r["Grid_Size"]) // int(r["Workgroup_Size"])
+            step = sorted(d for d in sums if "ens_step_kernel" in name[d])
+            red = [sums[d] for d in sums if "ens_reduce_kernel" in name[d]]
+            g = grid[step[-1]]
+            vals = [sums[d] for d in step if grid[d] == g]
+            kname = name[step[-1]].split("(")[0].replace("void ", "")
+            per[kind] = (g, statistics.median(vals), statistics.median(red) if red else 0.0, kname, len(vals))
+        g, f, fr, kname, nd = per["fetch"]
+        rb = f * 1024 * scale["fetch"]
+        wb = per["write"][1] * 1024 * scale["write"]
+        out[pname] = {"kernel": kname, "grid_blocks": g, "dispatches": nd,
+                      "read_B_per_node": round(rb / nodes, 3), "write_B_per_node": round(wb / nodes, 3),
+                      "traffic_bytes_per_launch": round(rb + wb),
+                      "reduce_read_B_per_node": round(fr * 1024 * scale["fetch"] / nodes, 3)}
     return out
 
 
@@ -839,6 +919,8 @@ def main():
     ap.add_argument("--burst", action="store_true", help=f"{SWEEP}: every request at t = 0")
     ap.add_argument("--no-sequential", action="store_true", help=f"{SWEEP}: skip the sequential FP64 solves")
     ap.add_argument("--sweep-fp16", action="store_true", help=f"{SWEEP}: FP16-shifted storage")
+    ap.add_argument("--pool-only", action="store_true",
+                    help=f"{SWEEP}: only the multi-case step kernel with all 64 solves resident (--steps timed)")
     ap.add_argument("--dr-steps", type=int, default=0, help="drag workloads: steps per run")
     ap.add_argument("--dr-avg-from", type=int, default=0, help="drag workloads: first averaged step")
     ap.add_argument("--dr-runs", default="", help="drag workloads: runs of this process, e.g. fp64:smooth,fp64:textured")
@@ -863,7 +945,8 @@ def main():
               "(--gpus must be 1)", file=sys.stderr)
         sys.exit(2)
     if args.traffic_summary:
-        print(json.dumps(run_traffic_summary(args), indent=1))
+        summ = run_ens_traffic_summary(args) if args.workload == SWEEP else run_traffic_summary(args)
+        print(json.dumps(summ, indent=1))
         return
     # the CPU baselines run before this process touches the GPU (the
     # reference runs as a child process)
@@ -890,7 +973,10 @@ def main():
         print(json.dumps(run_placement_sweep(args)))
         return
     if args.workload == SWEEP:
-        print(json.dumps(run_sweep(args)))
+        if args.pool_only:
+            print(json.dumps(run_pool_only(args)))
+        else:
+            print(json.dumps(run_sweep(args)))
         return
     if args.workload in DRAG:
         print(json.dumps(run_drag(args.workload, args)))

@@ -82,3 +82,32 @@ def test_roofline_fields():
     rf = bench.roofline_of(r, 60e9, "step_kernel<float,false,true,true>")
     assert rf["frac"] == 0.625 and rf["frac_of_copy_ceiling"] == round(5000.0 / bench.CEILING_GBS, 4)
     assert rf["traffic_gbs"] == 6000.0
+
+
+def test_ensemble_traffic_summary_from_counter_files(tmp_path):
+    """The multi-case step kernel's record (bench.py --workload
+    texture_sweep_64 --traffic-summary): the grid of the last step dispatch,
+    bytes per node of the 64-solve pool, and the record reduction's reads."""
+    nodes = 231473152  # the seeded 64-request pool (dnsserve_generate, seed 5)
+    k = "void lbm::ens_step_kernel<float, true>(lbm::EnsMember const*, lbm::EnsTile const*, int, int, int)"
+    rk = "lbm::ens_reduce_kernel(lbm::EnsMember const*, int)"
+    fk = 86 * nodes / 1024 / 2
+    wk = 78 * nodes / 1024
+    write_csv(tmp_path / "f.csv", [(k, 512 * 256, 256, "FETCH_SIZE", 2 * fk), (k, 768 * 256, 256, "FETCH_SIZE", fk),
+                                   (rk, 444 * 1024, 1024, "FETCH_SIZE", 2.5 * nodes / 1024 / 2),
+                                   (k, 768 * 256, 256, "FETCH_SIZE", fk)])
+    write_csv(tmp_path / "w.csv", [(k, 512 * 256, 256, "WRITE_SIZE", wk), (k, 768 * 256, 256, "WRITE_SIZE", wk),
+                                   (k, 768 * 256, 256, "WRITE_SIZE", wk)])
+    copy = "void copy_f4(float4 const*, float4*, unsigned long)"
+    write_csv(tmp_path / "cf.csv", [(copy, 262144, 256, "FETCH_SIZE", 2 * 1024 * 1024)] * 3)
+    write_csv(tmp_path / "cw.csv", [(copy, 262144, 256, "WRITE_SIZE", 4 * 1024 * 1024)] * 3)
+    arg = ",".join(str(tmp_path / n) for n in ("f.csv", "w.csv", "cf.csv", "cw.csv"))
+    out = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--workload", "texture_sweep_64",
+                          "--traffic-summary", arg], capture_output=True, text=True, check=True, timeout=120).stdout
+    rec = json.loads(out)
+    assert rec["nodes"] == nodes and rec["fetch_scale"] == 2.0 and rec["write_scale"] == 1.0
+    ent = rec["fp32"]
+    assert ent["grid_blocks"] == 768 and ent["dispatches"] == 2
+    assert abs(ent["read_B_per_node"] - 86.0) < 1e-3 and abs(ent["write_B_per_node"] - 78.0) < 1e-3
+    assert abs(ent["reduce_read_B_per_node"] - 2.5) < 1e-3
+    assert abs(ent["traffic_bytes_per_launch"] - 164 * nodes) < 1e-6 * 164 * nodes
