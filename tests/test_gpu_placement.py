@@ -13,7 +13,9 @@ populations and logs of an unplaced one at the same launch shape.
   ensemble, FP32 and FP16, created while the arenas are reserved (each
   member's two buffers ranges of blocks in different memory halves) and
   stepped 37 times, against the same members on plain allocations: states
-  and log rows bitwise; after the release every block is freed.
+  and log rows bitwise; a second reservation while the members hold ranges
+  grows each half until the request is free; after the release every
+  block is freed.
 """
 import os
 import sys
@@ -108,7 +110,13 @@ def _ensemble_run(precision, arenas):
         ms = _members(precision)
         if arenas:
             # every member's two buffers are ranges of the arenas
-            assert dnsgpu.pair_arena_stats(0)[2] >= sum(2 * m.X * m.Y * m.Z * 19 * 4 for m in ms)
+            used = dnsgpu.pair_arena_stats(0)[2]
+            assert used >= sum(2 * m.X * m.Y * m.Z * 19 * 4 for m in ms)
+            # a second reservation counts the bytes not held by the members:
+            # 8 GiB free per half needs a second block in each
+            got = dnsgpu.reserve_pair_arenas(8 << 30)
+            b0, b1, used2 = dnsgpu.pair_arena_stats(0)
+            assert got >= 8 << 30 and b0 >= 2 and b1 >= 2 and used2 == used
         ens = dnsgpu.Ensemble(precision=precision)
         for m in ms:
             ens.add(m)
