@@ -78,6 +78,8 @@ struct dnsens {
     std::vector<FinalizeArgs> h_fin;
     std::vector<EnsTile> h_tiles;
     int ntiles, nchunks, zmax;
+    int zslice;    // planes of one tile's plane sums (the largest ens_plane_rows)
+    bool pieces;   // some member is in z-pieces (above 512 rows)
     bool shl_lds;  // the step kernel keeps the members' shifts in LDS
     size_t lds;
     int resident;       // step-kernel blocks resident at once for this lds
@@ -176,11 +178,15 @@ static int find_member(const dnsens* e, const dnsgpu* h)
     return -1;
 }
 
-static int tiles_of(const dnsgpu* h)
+// Column tiles of a member, and its tiles (records): one per column tile and
+// z-piece.
+static int col_tiles_of(const dnsgpu* h)
 {
     const int cpt = ens_tile_cols(h->g.ncols);
     return (h->g.ncols + cpt - 1) / cpt;
 }
+
+static int tiles_of(const dnsgpu* h) { return col_tiles_of(h) * h->g.nzp; }
 
 // Bytes a member adds on top of its handle: its tile records.
 static size_t extra_bytes(const dnsgpu* h) { return sizeof(double) * (size_t)tiles_of(h) * ens_record(h->g.Z); }
@@ -193,9 +199,10 @@ extern "C" size_t dnsens_member_bytes(const dnsgpu* h)
 extern "C" size_t dnsens_footprint(const dns_case* c)
 {
     const size_t fp = dnsgpu_footprint(c);
-    if (!fp || c->zdv > kEnsZmax) return 0;
+    if (!fp) return 0;
     const int ncols = c->xdv * c->ydv, cpt = ens_tile_cols(ncols);
-    return fp + sizeof(double) * (size_t)((ncols + cpt - 1) / cpt) * ens_record(c->zdv);
+    const int pieces = z_pieces((c->zdv + kWave - 1) / kWave);
+    return fp + sizeof(double) * (size_t)((ncols + cpt - 1) / cpt) * pieces * ens_record(c->zdv);
 }
 
 extern "C" int dnsens_add(dnsens* e, dnsgpu* h)
@@ -210,9 +217,6 @@ extern "C" int dnsens_add(dnsens* e, dnsgpu* h)
     if (h->owner) return set_err(DNSGPU_EARG, "handle is a member of another ensemble");
     if ((int)e->mem.size() >= DNSENS_MAX_MEMBERS)
         return set_err(DNSGPU_EARG, "ensemble holds %d members already", DNSENS_MAX_MEMBERS);
-    if (h->g.Z > kEnsZmax)
-        return set_err(DNSGPU_EARG, "member zdv %d above the ensemble's %d (per-wave plane sums in LDS)", h->g.Z,
-                       kEnsZmax);
     const size_t need = dnsens_member_bytes(h);
     if (e->used + need > e->budget)
         return set_err(DNSGPU_ENOMEM, "member needs %zu bytes, %zu of %zu in use", need, e->used, e->budget);
@@ -293,9 +297,12 @@ extern "C" int dnsens_bind_thread(const dnsens* e)
 }
 
 template <typename T>
-static int step_occupancy(bool shl, size_t lds, int* per_cu)
+static int step_occupancy(bool shl, bool pieces, size_t lds, int* per_cu)
 {
-    if (shl)
+    if (pieces)
+        HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, ens_step_kernel<T, false, true>,
+                                                            kWave * kEnsWaves, lds));
+    else if (shl)
         HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, ens_step_kernel<T, true>, kWave * kEnsWaves, lds));
     else
         HIPCHK(hipOccupancyMaxActiveBlocksPerMultiprocessor(per_cu, ens_step_kernel<T, false>, kWave * kEnsWaves, lds));
@@ -317,7 +324,8 @@ static int upload_tables(dnsens* e)
     e->h_mem.resize(nm);
     e->h_fin.resize(nm);
     e->h_tiles.clear();
-    int chunk = 0, zmax = 0;
+    int chunk = 0, zmax = 0, zslice = 0;
+    bool pieces = false;
     for (int i = 0; i < nm; i++) {
         Member& m = e->mem[i];
         dnsgpu* h = m.h;
@@ -329,21 +337,28 @@ static int upload_tables(dnsens* e)
         d.planes = h->planes;
         d.chunk0 = chunk;
         d.nchunks = (ens_record(h->g.Z) + kWave - 1) / kWave;
+        d.pieces = h->g.nzp;
+        d.cpp = h->g.cpp;
         chunk += d.nchunks;
         FinalizeArgs F = fin_args(h, true, h->ring, 0, 0);
         F.cntr_dev = 1;
         e->h_fin[i] = F;
-        for (int t = 0; t < m.ntiles; t++) {
-            EnsTile tl{};
-            tl.m = i;
-            // columns t, t + S, ... below ncols: at most ens_tile_cols of them
-            tl.col0 = t;
-            tl.stride = m.ntiles;
-            tl.ncol = (h->g.ncols - t + m.ntiles - 1) / m.ntiles;
-            tl.lt = t;
-            e->h_tiles.push_back(tl);
-        }
+        const int S = col_tiles_of(h), P = h->g.nzp;
+        for (int t = 0; t < S; t++)
+            for (int pc = 0; pc < P; pc++) {
+                EnsTile tl{};
+                tl.m = i;
+                // columns t, t + S, ... below ncols: at most ens_tile_cols of them
+                tl.col0 = t;
+                tl.stride = S;
+                tl.ncol = (h->g.ncols - t + S - 1) / S;
+                tl.lt = t * P + pc;
+                tl.piece = pc;
+                e->h_tiles.push_back(tl);
+            }
         zmax = std::max(zmax, h->g.Z);
+        zslice = std::max(zslice, ens_plane_rows(h->g.Z));
+        pieces |= P > 1;
     }
     e->ntiles = (int)e->h_tiles.size();
     e->nchunks = chunk;
@@ -382,13 +397,18 @@ static int upload_tables(dnsens* e)
         // the host tables are rewritten by the next upload
         HIPCHK(hipStreamSynchronize(e->stream));
     }
-    if (zmax != e->zmax || e->resident == 0) {
+    if (zmax != e->zmax || pieces != e->pieces || e->resident == 0) {
         e->zmax = zmax;
-        e->shl_lds = ens_lds_bytes(std::max(zmax, 1), true) <= kEnsLdsMax;
-        e->lds = ens_lds_bytes(std::max(zmax, 1), e->shl_lds);
+        e->zslice = std::max(zslice, 1);
+        e->pieces = pieces;
+        // the shifts in LDS when they fit (never with z-pieces: a member of
+        // more than 512 rows), the plane sums of a tile always
+        e->shl_lds = !pieces && ens_lds_bytes(e->zslice, std::max(zmax, 1), true) <= kEnsLdsMax;
+        e->lds = ens_lds_bytes(e->zslice, std::max(zmax, 1), e->shl_lds);
         int per = 0;
-        int rc = with_storage(e->precision,
-                              [&](auto t) { return step_occupancy<decltype(t)>(e->shl_lds, e->lds, &per); });
+        int rc = with_storage(e->precision, [&](auto t) {
+            return step_occupancy<decltype(t)>(e->shl_lds, e->pieces, e->lds, &per);
+        });
         if (rc) return rc;
         e->resident = std::max(1, per) * e->ncu;
         e->layout_changed = true;
@@ -407,12 +427,16 @@ static int upload_tables(dnsens* e)
 template <typename T>
 static void enqueue_step_only(dnsens* e, int phase)
 {
-    if (e->shl_lds)
+    const int zs = e->zslice, zm = std::max(e->zmax, 1);
+    if (e->pieces)
+        hipLaunchKernelGGL((ens_step_kernel<T, false, true>), dim3(e->grid), dim3(kWave * kEnsWaves), e->lds,
+                           e->stream, e->d_mem, e->d_tiles, e->ntiles, zs, zm, phase);
+    else if (e->shl_lds)
         hipLaunchKernelGGL((ens_step_kernel<T, true>), dim3(e->grid), dim3(kWave * kEnsWaves), e->lds, e->stream,
-                           e->d_mem, e->d_tiles, e->ntiles, std::max(e->zmax, 1), phase);
+                           e->d_mem, e->d_tiles, e->ntiles, zs, zm, phase);
     else
         hipLaunchKernelGGL((ens_step_kernel<T, false>), dim3(e->grid), dim3(kWave * kEnsWaves), e->lds, e->stream,
-                           e->d_mem, e->d_tiles, e->ntiles, std::max(e->zmax, 1), phase);
+                           e->d_mem, e->d_tiles, e->ntiles, zs, zm, phase);
 }
 
 // k steps of every member: step, reduction, finisher per step; log row of

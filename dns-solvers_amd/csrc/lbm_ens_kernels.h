@@ -35,18 +35,27 @@ __host__ __device__ inline int ens_tile_cols(int ncols) { return ncols >= 32768 
 // the kNG global sums.
 __host__ __device__ inline int ens_record(int Z) { return kNQ * Z + kNG; }
 
-// LDS of the step kernel: each wave's private per-plane sums and, with
-// SHL, a copy of its member's [3][Z] plane-mean shifts (read by every
-// column of the tile: from LDS about 5% faster in FP16 storage).
-__host__ __device__ inline size_t ens_lds_bytes(int zmax, bool shl)
+// A member taller than 512 rows is split into z-pieces as in the
+// single-case step (z_pieces): each of its column tiles is one tile per
+// piece, with a record of its own.  Planes of one tile's plane sums:
+__host__ __device__ inline int ens_plane_rows(int Z)
 {
-    return sizeof(double) * kEnsWaves * (kNQ + (shl ? 3 : 0)) * (size_t)zmax;
+    const int nzc = (Z + kWave - 1) / kWave, P = z_pieces(nzc);
+    return P == 1 ? Z : (nzc + P - 1) / P * kWave;
+}
+
+// LDS of the step kernel: each wave's private per-plane sums (zslice planes:
+// the largest ens_plane_rows of the members) and, with SHL, a copy of its
+// member's [3][Z] plane-mean shifts (read by every column of the tile: from
+// LDS about 5% faster in FP16 storage).
+__host__ __device__ inline size_t ens_lds_bytes(int zslice, int zmax, bool shl)
+{
+    return sizeof(double) * kEnsWaves * ((size_t)kNQ * zslice + (shl ? 3 * (size_t)zmax : 0));
 }
 
 // Largest dynamic LDS the step kernel asks for (the shifts stay in global
-// memory above it) and the tallest member it takes.
+// memory above it): the plane sums of a 512-row piece fit it.
 constexpr size_t kEnsLdsMax = 65536;
-constexpr int kEnsZmax = (int)(kEnsLdsMax / (sizeof(double) * kEnsWaves * kNQ));
 
 // One member as the kernels see it.
 struct EnsMember {
@@ -55,15 +64,18 @@ struct EnsMember {
     double* planes;  // [kNQ*Z + kNG] reduced sums (the finisher's input)
     int chunk0;      // first 64-row chunk of this member in the reduction's chunk space
     int nchunks;     // ceil(ens_record(Z) / 64)
+    int pieces;      // z-pieces per column (z_pieces): records lt * pieces + p
+    int cpp;         // 64-row chunks per piece
 };
 
 struct EnsTile {
     int m;     // member
     int col0;  // first column
     int ncol;  // columns
-    int lt;    // tile index within the member (record slot)
-    int stride;  // column stride (the member's tile count)
-    int pad[3];
+    int lt;    // record slot within the member (column tile * pieces + piece)
+    int stride;  // column stride (the member's column-tile count)
+    int piece;   // z-piece of the columns (0 below 513 rows)
+    int pad[2];
 };
 
 // Orders the wave's LDS accesses across lanes (the zeroing and record
@@ -78,16 +90,19 @@ __device__ __forceinline__ void wave_lds_sync()
 // The fused step (step_column) over every tile of every member, one wave
 // per tile.  phase selects the buffer direction: even steps read a.src,
 // odd steps a.dst.
-template <typename T, bool SHL>
+// PIECES: some member is taller than 512 rows (its tiles cover one z-piece
+// each); without it every tile covers whole columns, with compile-time
+// piece bounds.
+template <typename T, bool SHL, bool PIECES = false>
 static __global__ __launch_bounds__(kWave* kEnsWaves) void ens_step_kernel(const EnsMember* __restrict__ mem,
                                                                           const EnsTile* __restrict__ tiles,
-                                                                          int ntiles, int zmax, int phase)
+                                                                          int ntiles, int zslice, int zmax, int phase)
 {
-    extern __shared__ double lds[];  // ens_lds_bytes(zmax, SHL)
+    extern __shared__ double lds[];  // ens_lds_bytes(zslice, zmax, SHL)
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-    double* mine = lds + (size_t)wave * (kNQ + (SHL ? 3 : 0)) * zmax;
-    double* shl = mine + kNQ * zmax;
+    double* mine = lds + (size_t)wave * ((size_t)kNQ * zslice + (SHL ? 3 * (size_t)zmax : 0));
+    double* shl = mine + kNQ * zslice;
     const int nw = gridDim.x * kEnsWaves;
     for (int t = blockIdx.x * kEnsWaves + wave; t < ntiles; t += nw) {
         const EnsTile tl = load_uniform(tiles + t);
@@ -108,7 +123,18 @@ static __global__ __launch_bounds__(kWave* kEnsWaves) void ens_step_kernel(const
         A.shift = load_uniform(&ma->shift);
         A.part = load_uniform(&ma->part);
         const int Z = A.g.Z;
-        for (int i = lane; i < kNQ * Z; i += kWave) mine[i] = 0.0;
+        // the tile's rows: chunks [c0, c1), plane sums of planes zbase .. zbase + pz - 1
+        int c0 = 0, c1 = A.g.nzc, pz = Z, zbase = 0;
+        if constexpr (PIECES) {
+            if (load_uniform(&mem[tl.m].pieces) > 1) {
+                const int cpp = load_uniform(&mem[tl.m].cpp);
+                c0 = tl.piece * cpp;
+                c1 = min(A.g.nzc, c0 + cpp);
+                pz = cpp * kWave;
+                zbase = c0 * kWave;
+            }
+        }
+        for (int i = lane; i < kNQ * pz; i += kWave) mine[i] = 0.0;
         if constexpr (SHL)
             for (int i = lane; i < 3 * Z; i += kWave) shl[i] = A.shift[i];
         wave_lds_sync();
@@ -118,10 +144,18 @@ static __global__ __launch_bounds__(kWave* kEnsWaves) void ens_step_kernel(const
         // all-zero band records, so no chunk takes the band branch
         for (int k = 0; k < tl.ncol; k++)
             step_column<T, false, true, true>(A, tl.col0 + k * tl.stride, lane, G, SHL ? shl : A.shift, mine, acc,
-                                              0, A.g.nzc, Z, 0);
+                                              c0, c1, pz, zbase);
         wave_lds_sync();
         double* rec = A.part + (long long)tl.lt * ens_record(Z);
-        for (int i = lane; i < kNQ * Z; i += kWave) rec[i] = mine[i];
+        if constexpr (PIECES) {
+            // this piece's planes only (the reduction reads no other plane of this record)
+            for (int i = lane; i < kNQ * pz; i += kWave) {
+                const int q = i / pz, z = zbase + (i - q * pz);
+                if (z < Z) rec[q * Z + z] = mine[i];
+            }
+        } else {
+            for (int i = lane; i < kNQ * Z; i += kWave) rec[i] = mine[i];
+        }
         double g[kNG] = {acc.jx, acc.mass, acc.mom, acc.sq};
 #pragma unroll
         for (int q = 0; q < kNG; q++) {
@@ -147,7 +181,10 @@ constexpr int kEnsReduceWaves = 16;  // waves per block of the record reduction
 // consecutive record rows (one per lane) of the member whose chunks
 // contain b.  Wave w sums the records t = w + 16 (4k + j) into accumulator
 // j in increasing k, the four accumulators are added pairwise, then the 16
-// wave sums in a fixed tree.  Reads are 512-byte record segments.
+// wave sums in a fixed tree.  Reads are 512-byte record segments.  A member
+// in z-pieces sums a plane row over the records of the piece holding the
+// plane only (record lt * pieces + piece, in increasing lt), its global rows
+// over every record.
 static __global__ __launch_bounds__(kWave* kEnsReduceWaves) void ens_reduce_kernel(const EnsMember* __restrict__ mem,
                                                                                   int nmem)
 {
@@ -164,15 +201,23 @@ static __global__ __launch_bounds__(kWave* kEnsReduceWaves) void ens_reduce_kern
     const double* part = load_uniform(&mem[m].a.part);
     double s[4] = {0.0, 0.0, 0.0, 0.0};
     if (r < nrec) {
+        // records first, first + stride, ... (cnt of them) hold row r
+        int first = 0, stride = 1, cnt = nt;
+        const int P = load_uniform(&mem[m].pieces);
+        if (P > 1 && r < kNQ * Z) {
+            first = (r % Z) / kWave / load_uniform(&mem[m].cpp);
+            stride = P;
+            cnt = nt / P;
+        }
         constexpr int S = kEnsReduceWaves;
         int t = wave;
-        for (; t + 3 * S < nt; t += 4 * S) {
+        for (; t + 3 * S < cnt; t += 4 * S) {
 #pragma unroll
-            for (int j = 0; j < 4; j++) s[j] += part[(long long)(t + j * S) * nrec + r];
+            for (int j = 0; j < 4; j++) s[j] += part[(long long)(first + (t + j * S) * stride) * nrec + r];
         }
 #pragma unroll
         for (int j = 0; j < 4; j++)
-            if (t + j * S < nt) s[j] += part[(long long)(t + j * S) * nrec + r];
+            if (t + j * S < cnt) s[j] += part[(long long)(first + (t + j * S) * stride) * nrec + r];
     }
     sh[wave][lane] = (s[0] + s[1]) + (s[2] + s[3]);
     __syncthreads();

@@ -403,7 +403,9 @@ size_t dnsens_member_bytes(const dnsgpu *h);
 /* The same for a case not yet created (0 when the case is refused). */
 size_t dnsens_footprint(const dns_case *c);
 
-/* Admit handle h (initialised or loaded, same precision).
+/* Admit handle h (initialised or loaded, same precision; any grid
+ * dnsgpu_create takes, a member above 512 rows in z-pieces as in the
+ * single-case step).
  * Returns DNSGPU_ENOMEM, with nothing changed, when its bytes do not fit
  * the budget, and DNSGPU_EARG when h is a member already (of this or of
  * another ensemble: a handle has one owner until dnsens_remove).  A state

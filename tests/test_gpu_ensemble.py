@@ -154,8 +154,8 @@ def test_members_match_the_oracle(forcing):
 def test_tall_member_matches_single_case_path():
     """A member taller than 292 rows makes the step kernel read the plane
     shifts from global memory instead of LDS: the same bitwise results
-    against the single-case path (constant F).  A case above 512 rows has
-    no member footprint (the ensemble does not take it)."""
+    against the single-case path (constant F).  A case above 1024 rows has
+    no member footprint (neither path takes it)."""
     print("case: tall member", flush=True)
     n, F = 12, 2e-5
     ens = dnsgpu.Ensemble(precision=dnsgpu.FP64)
@@ -169,7 +169,7 @@ def test_tall_member_matches_single_case_path():
         ref.step(n + 1, cntr0=0)
         assert np.array_equal(ch.save_aos()[0], ref.save_aos()[0]), shape
         ref.close()
-    assert dnsgpu.member_footprint(8, 8, 513, dnsgpu.FP64) == 0
+    assert dnsgpu.member_footprint(8, 8, 1025, dnsgpu.FP64) == 0
     for ch in members:
         ch.close()
     ens.close()

@@ -13,8 +13,9 @@ oracle (pinned bitwise to the reference):
   * FP32-shifted storage with riblets, constant Q, against FP64: the bars
     of the shorter columns (1e-5 in the populations, 1e-3 of the force).
 The step grid is a multiple of the piece count; two rows per lane (FP16)
-takes one piece only and falls back to one row; the ensemble does not take
-members above 512 rows.
+takes one piece only and falls back to one row; an ensemble member above 512
+rows runs in z-pieces too (one tile and record per piece), bitwise the
+single-case path under constant force and the same alone and in a mix.
 
 Columns of 6 or 8 chunks (321-384, 449-512 rows) may also run as two
 pieces (dnsgpu_set_step_pieces; the default for FP64 and FP16 storage): the
@@ -197,16 +198,47 @@ def test_tall_launch_shapes():
         assert np.array_equal(ft, ref)
 
 
-def test_ensemble_refuses_tall_member():
-    print("case: ensemble with a 640-row member", flush=True)
-    assert dnsgpu.member_footprint(4, 4, 640, dnsgpu.FP32) == 0
-    ens = dnsgpu.Ensemble(precision=dnsgpu.FP32)
-    ch = dnsgpu.Channel(4, 4, 640, Rebs=60.0, ub=0.05, CFL=0.25, precision=dnsgpu.FP32)
-    ch.init_rest()
-    with pytest.raises(dnsgpu.DnsGpuError, match="above the ensemble"):
-        ens.add(ch)
-    ens.close()
-    ch.close()
+@pytest.mark.parametrize("precision", [dnsgpu.FP64, dnsgpu.FP32])
+def test_tall_member_in_an_ensemble(precision):
+    """A member of 640 rows (two z-pieces: one tile and record per piece)
+    beside a short one: bitwise the single-case path's state under constant
+    force, and the same state alone and in the mix."""
+    print(f"case: ensemble with a 640-row member, precision {precision}", flush=True)
+    n, F = 12, 2e-5
+    shapes = [(4, 6, 640), (8, 8, 40)]
+
+    def member(shape):
+        X, Y, Z = shape
+        ch = dnsgpu.Channel(X, Y, Z, Rebs=60.0, ub=0.05, CFL=0.25, TSTR=5, forcing=dnsgpu.CONST_F, force=F,
+                            precision=precision)
+        m = fields.posts_mask(X, Y)
+        ch.set_walls(m, m)
+        ch.init_field(dnsgpu.INIT_POISEUILLE, uc=0.06, amp=0.05, seed=7)
+        return ch
+
+    assert dnsgpu.member_footprint(4, 6, 640, precision) > dnsgpu.footprint(4, 6, 640, precision)
+    mix = dnsgpu.Ensemble(precision=precision)
+    members = [member(s) for s in shapes]
+    for ch in members:
+        mix.add(ch)
+    mix.step(n)
+    alone = dnsgpu.Ensemble(precision=precision)
+    solo = member(shapes[0])
+    alone.add(solo)
+    alone.step(n)
+    for shape, ch in zip(shapes, members):
+        ref = member(shape)
+        ref.step(n + 1, cntr0=0)
+        assert np.array_equal(ch.save_aos()[0], ref.save_aos()[0]), shape
+        ref.close()
+    assert np.array_equal(solo.save_aos()[0], members[0].save_aos()[0])
+    ra, rm = alone.rows(solo, n), mix.rows(members[0], n)
+    for k in dnsgpu.LOG_FIELDS:
+        assert np.array_equal(ra[k], rm[k]), k
+    mix.close()
+    alone.close()
+    for ch in members + [solo]:
+        ch.close()
 
 
 @pytest.mark.parametrize("Z,precision,default", [(384, dnsgpu.FP64, 2), (512, dnsgpu.FP32, 1),
