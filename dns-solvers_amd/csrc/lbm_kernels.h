@@ -958,9 +958,10 @@ __device__ __forceinline__ void step_column_of(const StepArgs& A, int col, int l
 }
 
 // R: rows per lane (1, or 2 for FP16 storage with an even Z).
-// PIECES: columns above 512 rows, split into A.g.nzp z-pieces (gridDim.x a
-// multiple of nzp: block b takes piece b % nzp of the columns of its group
-// b / nzp); without it the whole column, with compile-time piece bounds.
+// PIECES: columns split into A.g.nzp z-pieces (always above 512 rows; at
+// 321-384 and 449-512 rows by launch choice), gridDim.x a multiple of nzp:
+// block b takes piece b % nzp of the columns of its group b / nzp; without
+// it the whole column, with compile-time piece bounds.
 template <typename T, bool FIRST, bool STATS, bool BANDS, int R = 1, bool PIECES = false>
 static __global__ __launch_bounds__(kWave* kMaxWaves) void step_kernel(StepArgs A)
 {

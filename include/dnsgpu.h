@@ -302,9 +302,9 @@ int dnsgpu_autotune(dnsgpu *h, int trial_steps, int *best_blocks, double *best_m
  * counterpart. */
 int dnsgpu_step_shape(const dnsgpu *h, int *blocks, int *waves_per_block, int *rows_per_lane);
 /* Sets the launch shape by hand (blocks are capped at what is resident at
- * once and, above 512 rows, rounded down to a multiple of the column's
- * z-pieces of at most 512 rows); rows_per_lane 2 needs FP16 storage and an
- * even zdv up to 512.  The state and the results do not depend on the
+ * once and rounded down to a multiple of the step's z-pieces, see
+ * dnsgpu_step_pieces); rows_per_lane 2 needs FP16 storage and an even zdv
+ * up to 512, and takes one z-piece.  The state and the results do not depend on the
  * shape beyond the summation order of the logged sums.  No reference
  * counterpart. */
 int dnsgpu_set_step_shape(dnsgpu *h, int blocks, int waves_per_block, int rows_per_lane);
