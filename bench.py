@@ -1059,8 +1059,9 @@ def main():
         out["cpu_baseline_mpi"] = cpu_mpi
     if not args.no_serving:
         # the serving figure: the FP32 sweep with every request at t = 0
-        # against the same solves one after another on the plain FP64 path
-        m = sweep_measure(dnsgpu.FP32, burst=True, sequential=True, kernel=False)
+        # against the same solves one after another on the plain FP64 path,
+        # and the roofline of its dominant kernel (the pool, every solve resident)
+        m = sweep_measure(dnsgpu.FP32, burst=True, sequential=True, kernel=True)
         m.pop("per_solve")
         out["serving"] = m
     print(json.dumps(out))
